@@ -1,0 +1,213 @@
+"""Benchmark of the simulation step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload plain-1m]
+
+One step is one NeuCor::run() with runAll = true and runSpeed = dt = 1 ms over
+the whole network (drive, neuron update, STDP, fan-out).  The network and its
+state are resident in HBM before the timed region starts.  Prints one JSON
+line with the throughput in synapse-updates/s (one synapse-update = one
+(synapse, step) pair), the roofline of the dominant kernel (HIP events around
+each of its launches inside the timed region) and the reference's own
+NeuCor::run() timed on this host's cores on a bounded sample.
+"""
+import argparse
+import json
+import math
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "synapse updates/s (% HBM roofline); served sims/s + p50/p99 vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def neuron_kernel_bytes(n_neurons, n_syn, c0, c1, steps):
+    """Algorithmic bytes per launch of the neuron-step kernel, from the
+    device counters over the timed steps:
+      16 B per neuron  (in-degree word 4, V read+write 8, lastFire read 4)
+       1 B per synapse (spike-latch byte scanned every step)
+       4 B per active slot-step (latched depolarisation read)
+      13 B per processed arrival (w read+write 8, last-arrival write 4,
+                                  latch clear 1)
+      20 B per fired neuron (lastFire write 4, firings read+write 8,
+                             fired-list entry 4, raster/compaction 4)"""
+    A = (c1["active_slot_steps"] - c0["active_slot_steps"]) / steps
+    R = (c1["arrivals"] - c0["arrivals"]) / steps
+    F = (c1["fired"] - c0["fired"]) / steps
+    return 16.0 * n_neurons + 1.0 * n_syn + 4.0 * A + 13.0 * R + 20.0 * F, {"A": A, "R": R, "F": F}
+
+
+def drive_fires(drive, ids, groups, n_corr_groups, steps, keep):
+    """Input fires of steps 1..steps for the inputs in `keep` (the device's
+    group draw restated on the host with the oracle's Philox)."""
+    from oracle import oracle as O
+    key = (drive.seed & 0xFFFFFFFF, drive.seed >> 32)
+    per = [[] for _ in range(steps)]
+    members = {}
+    for i, g in zip(ids.tolist(), groups.tolist()):
+        if i in keep:
+            members.setdefault(g, []).append(i)
+    for s in range(1, steps + 1):
+        k = s % drive.period
+        for g, mem in members.items():
+            amp = drive.amp_corr if g < n_corr_groups else drive.amp_ind
+            p = drive.p0 * (1.0 + amp * math.sin(2.0 * math.pi * k / drive.period))
+            thr = 0 if p <= 0 else (0xFFFFFFFF if p >= 1 else int(math.floor(p * 4294967296.0)))
+            u = int(O.philox((s, g, 4, 0), key)[0])
+            if u < thr:
+                per[s - 1].extend(mem)
+    return [sorted(x) for x in per]
+
+
+def cpu_baseline(net, drive, budget_s=12.0, sample_neurons=40_000):
+    """The reference's NeuCor::run() (oracle/_ref/ref_harness, built from the
+    reference's src/NeuCor.cpp) on the first `sample_neurons` neurons of the
+    workload (a compact region, ids follow a Morton curve) with their
+    internal synapses; single thread."""
+    from oracle import oracle as O
+    from oracle import ref as R
+    ex = net.export()
+    M = min(sample_neurons, len(ex["in_ptr"]) - 1)
+    ptr = ex["in_ptr"][:M + 1]
+    src = ex["in_src"][:ptr[M]]
+    keep = src < M
+    rows = np.repeat(np.arange(M), np.diff(ptr))[keep]
+    sub = {"in_src": src[keep].copy(), "w": ex["w"][:ptr[M]][keep].copy(),
+           "delay": ex["delay"][:ptr[M]][keep].copy(),
+           "in_ptr": np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=M))]).astype(np.int64)}
+    S = len(sub["in_src"])
+    ids, groups = net.select_inputs(drive)
+    ncorr = drive.n_groups if drive.n_correlated > 0 else 0
+    cores = 1
+    cpu_model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu_model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    if R.available():
+        probe = R.run_bench(sub, 3, inputs=drive_fires(drive, ids, groups, ncorr, 3, set(range(M))))
+        per_step = max(probe["run_s"] / 3, 1e-6)
+        steps = int(max(5, min(2000, budget_s / per_step)))
+        res = R.run_bench(sub, steps, inputs=drive_fires(drive, ids, groups, ncorr, steps, set(range(M))))
+        return {"value": S * steps / res["run_s"], "unit": "synapse-updates/s", "cores": cores,
+                "kind": "reference",
+                "sample": "reference NeuCor::run() (runAll, runSpeed = 1 ms, STDP on, sign flags repaired) "
+                          "on the first %d neurons of the workload network and their %d internal synapses, "
+                          "%d steps after construction; 1 thread on %s (nproc %d)"
+                          % (M, S, steps, cpu_model, os.cpu_count() or 0)}
+    o = O.Sim(sub)
+    o.set_drive(ids[ids < M], groups[ids < M], ncorr, drive.period, drive.p0, drive.amp_corr, drive.amp_ind,
+                0.0, drive.seed)
+    t0 = time.perf_counter()
+    o.step(3, cap=M * 3)
+    per_step = max((time.perf_counter() - t0) / 3, 1e-6)
+    steps = int(max(5, min(5000, budget_s / per_step)))
+    t0 = time.perf_counter()
+    o.step(steps, cap=M * steps)
+    dt = time.perf_counter() - t0
+    return {"value": S * steps / dt, "unit": "synapse-updates/s", "cores": cores, "kind": "port",
+            "sample": "C oracle (clocked restatement of NeuCor::run) on the first %d neurons and %d internal "
+                      "synapses, %d steps; 1 thread on %s" % (M, S, steps, cpu_model)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10_000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--workload", default="plain-1m")
+    ap.add_argument("--chunk", type=int, default=500)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU: every box in this setup has a single MI355X; "
+              "independent networks shard across GPUs without communication, see DESIGN.md",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import neurocorrelation_amd as nc
+    from neurocorrelation_amd import workloads as W
+
+    t0 = time.perf_counter()
+    net, drive, params, _ = W.build(args.workload, seed=0)
+    info = net.info()
+    N, S = info["n"], info["n_syn"]
+    sim = nc.Simulation.for_network(net)
+    nid = sim.add(net, params, drive)
+    setup_s = time.perf_counter() - t0
+    print("[bench] %s: %d neurons, %d synapses, setup %.1fs" % (args.workload, N, S, setup_s),
+          file=sys.stderr, flush=True)
+
+    done = 0
+    while done < args.warmup:
+        k = min(args.chunk, args.warmup - done)
+        sim.step(k)
+        done += k
+    sim.synchronize()
+    c0 = sim.counters()
+    sim.set_kernel_timing(1 << 1)  # HIP events around every neuron-step launch
+    t_start = time.perf_counter()
+    done = 0
+    while done < args.steps:
+        k = min(args.chunk, args.steps - done)
+        sim.step(k)
+        done += k
+    sim.synchronize()
+    elapsed = time.perf_counter() - t_start
+    c1 = sim.counters()
+    launches, k_ms = sim.kernel_timing(1)
+    sim.set_kernel_timing(0)
+    print("[bench] timed %d steps in %.3fs" % (args.steps, elapsed), file=sys.stderr, flush=True)
+
+    value = S * args.steps / elapsed
+    bytes_launch, ev = neuron_kernel_bytes(N, S, c0, c1, args.steps)
+    avg_launch_s = (k_ms / 1000.0) / max(launches, 1)
+    achieved = bytes_launch / avg_launch_s / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
+    if os.path.exists(tpath):
+        traffic = json.load(open(tpath)).get("neuron_step_bytes_per_launch")
+    rate = ev["F"] / N
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            cpu = cpu_baseline(net, drive)
+        except Exception as e:  # reported, never fatal for the GPU number
+            print("[bench] cpu baseline failed: %r" % (e,), file=sys.stderr)
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "synapse-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed * 1000.0 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": args.workload, "neurons": N, "synapses": S, "dt_ms": params.dt,
+                   "stdp": True, "seed": 0, "fired_per_step": ev["F"], "rate_per_step": rate,
+                   "arrivals_per_step": ev["R"], "active_slots_per_step": ev["A"]},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_neuron", "algorithmic_bytes_per_launch": bytes_launch,
+                     "avg_launch_us": avg_launch_s * 1e6, "launches": launches},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

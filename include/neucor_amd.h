@@ -1,0 +1,254 @@
+/*
+ * neucor_amd.h -- C ABI of the MI355X NeuroCorrelation simulation engine.
+ *
+ * The engine replaces the internals of the reference's simulation core
+ * (class NeuCor, src/NeuCor.h:36-138, src/NeuCor.cpp).  Every entry point
+ * below names the reference interface it stands in for.  Plain pointers and
+ * sizes only; no exceptions cross this boundary; every call returns an
+ * nc_status and nc_last_error() describes the last failure on the thread.
+ *
+ * Objects:
+ *   nc_net  -- a network on the host: neuron positions and, per neuron, the
+ *              incoming synapses in ascending source id (the order of the
+ *              std::map Neuron::inSynapses, src/NeuCor.h:212).
+ *   nc_sim  -- a device-resident engine holding one or more networks.  The
+ *              plain path is an nc_sim with one network (id 0); the batched
+ *              and served forms add many networks of uneven size to one
+ *              nc_sim and advance them together.
+ *
+ * Threading: one host thread per nc_sim at a time; each nc_sim owns one HIP
+ * stream; read calls synchronise that stream.
+ */
+#ifndef NEUCOR_AMD_H
+#define NEUCOR_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum nc_status {
+  NC_OK = 0,
+  NC_ERR_ARG = 1,       /* bad argument (std::invalid_argument-like)            */
+  NC_ERR_RANGE = 2,     /* id out of range (std::out_of_range in the reference)  */
+  NC_ERR_HIP = 3,       /* a HIP runtime call failed                             */
+  NC_ERR_OOM = 4,       /* device or host allocation failed                      */
+  NC_ERR_CAPACITY = 5,  /* the nc_sim has no room for the network (admission)    */
+  NC_ERR_STATE = 6,     /* call not valid in the current state                   */
+  NC_ERR_NO_DEVICE = 7  /* no usable GPU                                          */
+} nc_status;
+
+typedef struct nc_net nc_net;
+typedef struct nc_sim nc_sim;
+
+/* ---------------------------------------------------------------------------
+ * Network construction
+ * ------------------------------------------------------------------------ */
+
+/* Distance wiring rule of the workloads (a cube of uniform-random positions;
+ * every ordered pair closer than `radius` is a synapse, in both directions,
+ * as Neuron::makeConnections does with radius 1.0, src/NeuCor.cpp:418-444).
+ * delay steps = clamp(ceil(delay_max * dist / radius), delay_min, delay_max);
+ * weight ~ U[w_min, w_max), negated with probability inhib_frac
+ * (src/NeuCor.cpp:471-475 uses [0.2, 1.0) and 0.2).  All draws come from
+ * Philox4x32-10 keyed by `seed`.  spatial_order = 1 numbers the neurons along
+ * a Morton curve of radius-sized cells. */
+typedef struct nc_wiring_spec {
+  uint32_t n_neurons;
+  uint32_t spatial_order;
+  uint64_t seed;
+  float side;
+  float radius;
+  uint32_t delay_min;
+  uint32_t delay_max;
+  float w_min;
+  float w_max;
+  float inhib_frac;
+  uint32_t n_threads; /* host threads for the build (0 = hardware concurrency) */
+} nc_wiring_spec;
+
+/* Replaces NeuCor::NeuCor(int) + Neuron::makeConnections
+ * (src/NeuCor.cpp:17-42, :418-444) for the workload wiring rule. */
+nc_status nc_net_generate(const nc_wiring_spec* spec, nc_net** out);
+
+/* Replaces NeuCor::createNeuron / NeuCor::createSynapse
+ * (src/NeuCor.cpp:154-195): an explicit network.  xyz may be NULL.
+ * Synapse k goes from[k] -> to[k]; duplicates of an existing (from,to) pair
+ * are ignored as createSynapse does (:189-191); delay in steps (>= 1);
+ * inhibitory may be NULL (then the sign of w decides, as setWeight, :576-579). */
+nc_status nc_net_from_edges(uint32_t n_neurons, const float* xyz,
+                            uint64_t n_syn, const uint32_t* to,
+                            const uint32_t* from, const float* w,
+                            const uint8_t* delay_steps,
+                            const uint8_t* inhibitory, nc_net** out);
+
+nc_status nc_net_info(const nc_net* net, uint32_t* n_neurons, uint64_t* n_syn,
+                      uint32_t* max_in_degree, uint32_t* max_delay);
+
+/* Copy out the network: in_ptr has n+1 entries; rows in ascending source. */
+nc_status nc_net_export(const nc_net* net, float* xyz, int64_t* in_ptr,
+                        uint32_t* in_src, float* w, uint8_t* delay_steps,
+                        uint8_t* inhibitory);
+
+void nc_net_free(nc_net* net);
+
+/* ---------------------------------------------------------------------------
+ * Simulation parameters and input drive
+ * ------------------------------------------------------------------------ */
+
+/* NeuCor's public parameters (src/NeuCor.h:68-73, defaults :18-29).
+ * dt is the step in ms (a power of two, 2^-8 .. 1): NeuCor::runSpeed with
+ * runAll = true.  learning_rate and the factors are read live each step
+ * (src/NeuCor.cpp:757-758); the trace decays are fixed at creation as in
+ * the reference (const members, src/NeuCor.h:236, :289). */
+typedef struct nc_params {
+  float dt;
+  float learning_rate;
+  float pre_factor;
+  float post_factor;
+  float pre_trace_decay;
+  float post_trace_decay;
+} nc_params;
+
+/* Device-generated drive (replaces InputFirer, src/NeuCor.cpp:305-345, and
+ * the background firing of NeuCor::run, :604-607, for the workloads).
+ * n_inputs neurons are chosen by seeded hash rank; the first n_correlated of
+ * them (by rank) form n_groups groups that each share one Bernoulli spike
+ * train, the rest fire independently.  p(s) = p0 * (1 + amp * sin(2 pi s /
+ * period)).  Input fires bypass threshold and refractory (InputFirer::run
+ * calls Neuron::fire directly, :326-331).  bg_prob forces a
+ * threshold-bypassing but refractory-respecting fire per neuron per step
+ * (Neuron::scheduleFire, :658-661, :683). */
+typedef struct nc_drive_spec {
+  uint32_t n_inputs;
+  uint32_t n_groups;
+  uint32_t n_correlated;
+  uint32_t period;
+  float p0;
+  float amp_corr;
+  float amp_ind;
+  float bg_prob;
+  uint64_t seed;
+} nc_drive_spec;
+
+/* The inputs nc_drive_spec selects on a network (ids ascending by rank) and
+ * their group index (independent inputs get their own group). */
+nc_status nc_net_select_inputs(const nc_net* net, const nc_drive_spec* drive,
+                               uint32_t* ids, uint32_t* groups);
+
+/* ---------------------------------------------------------------------------
+ * Engine
+ * ------------------------------------------------------------------------ */
+
+/* Capacity of an nc_sim: total neurons, synapses and networks it may hold. */
+typedef struct nc_sim_spec {
+  int32_t device;
+  uint64_t max_neurons;
+  uint64_t max_synapses;
+  uint32_t max_nets;
+  uint32_t raster_capacity; /* spike records kept on device (0 = no raster) */
+  uint64_t max_slots;       /* synapse slots incl. layout padding (0 = derive) */
+} nc_sim_spec;
+
+nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out);
+
+/* Exact device layout size of a network: neuron positions (a multiple of
+ * 64) and synapse slots including padding, to size an nc_sim_spec. */
+nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, uint64_t* slots);
+void nc_sim_destroy(nc_sim* sim);
+
+/* Bytes a network of this size takes on the device, and the free HBM the
+ * admission policy sees (hipMemGetInfo). */
+nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64_t* bytes);
+nc_status nc_sim_free_bytes(const nc_sim* sim, uint64_t* free_bytes);
+
+/* Upload a network; its state starts as NeuCor's constructors leave it
+ * (V = -70, never fired, no spike in flight).  drive may be NULL.
+ * steps_to_run = 0 means "run until told otherwise".  Fails with
+ * NC_ERR_CAPACITY when the nc_sim has no room. */
+nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params* params,
+                     const nc_drive_spec* drive, uint32_t steps_to_run,
+                     int32_t* net_id);
+nc_status nc_sim_remove(nc_sim* sim, int32_t net_id);
+
+/* Replaces NeuCor::run() (src/NeuCor.cpp:583-617) with runAll = true and
+ * runSpeed = dt: advance every active network by up to n_steps steps
+ * (networks with fewer steps left stop early). */
+nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps);
+
+/* Steps each network has completed and has left (0 = finished when it was
+ * added with a step budget; UINT32_MAX = unbounded). */
+nc_status nc_sim_progress(nc_sim* sim, int32_t net_id, uint32_t* steps_done,
+                          uint32_t* steps_left);
+
+/* Replaces NeuCor::getTime() (src/NeuCor.cpp:225): steps_done * dt. */
+nc_status nc_get_time(nc_sim* sim, int32_t net_id, float* time_ms);
+
+/* Live parameters (NeuCor::learningRate, presynapticFactor,
+ * postsynapticFactor: src/NeuCor.h:71-73). */
+nc_status nc_set_params(nc_sim* sim, int32_t net_id, float learning_rate,
+                        float pre_factor, float post_factor);
+
+/* Host-scheduled forced fires for the next steps (replaces the events that
+ * InputFirer::schedule, src/NeuCor.cpp:333-345, and the background loop,
+ * :604-607, push onto the simulation queue).  Event k fires neuron ids[k]
+ * at step offset step_off[k] (1 = the next step) of net_id; kind[k] is 0 for
+ * an input fire (Neuron::fire, bypasses refractory) and 1 for a background
+ * fire (scheduledFireTime, respects refractory).  Consumed by nc_sim_step. */
+nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_events,
+                         const uint32_t* step_off, const uint32_t* ids,
+                         const uint8_t* kind);
+
+/* State read-out, by neuron id / by synapse in nc_net_export order.
+ * Neuron::potential (src/NeuCor.cpp:456), Neuron::lastFire (src/NeuCor.h:226,
+ * as a step; NC_NEVER_STEP = never), Neuron::firings (:645). */
+#define NC_NEVER_STEP (-(1 << 30))
+nc_status nc_read_potential(nc_sim* sim, int32_t net_id, float* out, uint32_t n);
+nc_status nc_read_last_fire(nc_sim* sim, int32_t net_id, int32_t* out, uint32_t n);
+nc_status nc_read_firings(nc_sim* sim, int32_t net_id, uint32_t* out, uint32_t n);
+
+/* Synapse::getWeight / setWeight (src/NeuCor.cpp:572-579). */
+nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, uint64_t n);
+nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* w, uint64_t n);
+
+/* Synapse spike state for the renderer's getPrePot/getPostPot
+ * (src/NeuCor.cpp:551-567): arrival step of the spike in flight (0 = none,
+ * Synapse::AP_fireTime == 0) and last arrival step (NC_NEVER_STEP = -inf). */
+nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t* arrival,
+                                int32_t* last_arrival, float* depol, uint64_t n);
+
+/* Spike raster (the renderer's raster plot reads Neuron::lastFire after each
+ * run, src/NeuCor_Renderer.cpp:1858-1860).  Returns up to cap (step, id)
+ * records logged since the last call, ordered by step then id. */
+nc_status nc_read_raster(nc_sim* sim, int32_t net_id, uint32_t* steps,
+                         uint32_t* ids, uint64_t cap, uint64_t* n_out);
+
+/* NeuCor::resetActivities (src/NeuCor.cpp:233-235): firings := 0. */
+nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
+
+/* Event counters since creation, summed over all networks:
+ * [0] steps*neurons, [1] synapse-steps (sum of S per step),
+ * [2] active slot-steps, [3] arrivals, [4] fired neurons,
+ * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes. */
+nc_status nc_read_counters(nc_sim* sim, uint64_t out[8]);
+
+/* Block until every step enqueued on the nc_sim's stream has finished. */
+nc_status nc_sim_synchronize(nc_sim* sim);
+
+/* Per-kernel timing (HIP events around each launch, on the engine's stream)
+ * for the kernels in kernel_mask (bit k: 0 drive, 1 neuron step, 2 post
+ * STDP, 3 fan-out; 0 = off); nc_read_kernel_timing returns launches and
+ * total milliseconds since the last nc_set_kernel_timing call. */
+nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask);
+nc_status nc_read_kernel_timing(nc_sim* sim, uint32_t kernel, uint64_t* launches,
+                                double* total_ms);
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* nc_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NEUCOR_AMD_H */

@@ -1,0 +1,61 @@
+"""Build the engine's shared library in-tree (gfx950).
+
+Host C++ sources are compiled with g++, HIP sources with hipcc
+(--offload-arch=gfx950), and both are linked by hipcc into
+neurocorrelation_amd/libneucor_amd.so.  Floating-point contraction is off
+everywhere: the step reproduces the reference's float/double rounding
+sequence operation by operation, and a fused multiply-add would change it.
+"""
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+OUT = os.path.join(HERE, "libneucor_amd.so")
+ARCH = os.environ.get("NC_OFFLOAD_ARCH", "gfx950")
+
+HIP_FLAGS = [
+    "--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-fno-fast-math", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-result",
+]
+CXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-fno-fast-math", "-Wall", "-pthread"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    err = "\n".join(l for l in r.stderr.splitlines() if "packed-fp32-ops" not in l)
+    if r.returncode != 0:
+        sys.stderr.write(" ".join(cmd) + "\n" + err + "\n")
+        raise RuntimeError("build step failed: " + cmd[0])
+    return err
+
+
+def build(verbose=False):
+    hipcc = shutil.which("hipcc") or "/opt/rocm/bin/hipcc"
+    bdir = os.path.join(HERE, "_build")
+    os.makedirs(bdir, exist_ok=True)
+    objs = []
+    for src in sorted(os.listdir(CSRC)):
+        path = os.path.join(CSRC, src)
+        obj = os.path.join(bdir, src + ".o")
+        if src.endswith(".cpp"):
+            cmd = ["g++"] + CXX_FLAGS + ["-c", path, "-o", obj]
+        elif src.endswith(".hip"):
+            cmd = [hipcc] + HIP_FLAGS + ["-c", path, "-o", obj]
+        else:
+            continue
+        msg = _run(cmd)
+        if verbose and msg.strip():
+            print(msg)
+        objs.append(obj)
+    tmp = OUT + ".tmp"
+    _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
+    os.replace(tmp, OUT)
+    return OUT
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

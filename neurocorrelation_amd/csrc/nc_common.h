@@ -1,0 +1,106 @@
+// Shared host/device definitions of the NeuroCorrelation MI355X engine.
+//
+// Model constants are the reference defaults (src/NeuCor.cpp:18-29 for the
+// network-level STDP parameters, :376-385 for the neuron, :469-485 for the
+// synapse).  Time is an integer step count s; simulated time is s*dt with
+// dt = 2^-m ms, so every time difference the reference takes in float is an
+// exact multiple of dt and every transcendental it evaluates becomes a table
+// indexed by a step difference (tables are built on the host from its libm).
+#pragma once
+#include <stdint.h>
+
+#if defined(__HIP__)
+#include <hip/hip_runtime.h>
+#define NC_HD __host__ __device__ __forceinline__
+#else
+#define NC_HD static inline
+#endif
+
+// ---- reference constants -------------------------------------------------
+#define NC_BASELEVEL   (-70.0f)   // Neuron::baselevel       src/NeuCor.cpp:376
+#define NC_THRESHOLD   (-55.0f)   // Neuron::threshold       src/NeuCor.cpp:377
+#define NC_RECHARGE    (0.5f)     // Neuron::recharge        src/NeuCor.cpp:378
+#define NC_AP_CUTOFF   (2.0f)     // Neuron::AP_cutoff       src/NeuCor.cpp:385
+#define NC_AP_H        (100.0f)   // AP shape                src/NeuCor.cpp:384
+#define NC_AP_DEPOLW   (0.3f)
+#define NC_AP_POLW     (0.6f)
+#define NC_AP_DELTAPOL (1.16f)
+#define NC_AP_DEPOLFAC (0.2f)
+#define NC_AP_DSTART   (1.0f)
+#define NC_SYN_DEPOL_MUL (52.0)   // Synapse::fire           src/NeuCor.cpp:730
+#define NC_AP_SPEED    (2.0f)     // Synapse::AP_speed       src/NeuCor.cpp:485
+#define NC_CHARGE_K    (0.9943)   // charge_insynapses       src/NeuCor.cpp:695
+#define NC_CHARGE_R    (0.3702)
+
+// "never" for lastFire (NaN in the reference) and lastSpikeArrival (-inf):
+// with s < 2^29 every difference s - NC_NEVER exceeds all table lengths and
+// all refractory / AP windows, which is exactly how NaN and -inf behave there.
+#define NC_NEVER (-(1 << 30))
+#define NC_MAX_STEPS (1 << 29)
+
+// ---- stamp codes -----------------------------------------------------------
+// A synapse carries at most one spike (src/NeuCor.cpp:728).  Its in-flight
+// state is one byte: 0 = idle, else (arrival_step mod 255) + 1.  The offset
+// o = s - arrival is recovered modulo 255; it is unambiguous while
+// max_delay + release_offset < 255 (checked on the host).
+NC_HD uint32_t nc_stamp_code(int32_t arrival_step) {
+  return (uint32_t)(arrival_step % 255) + 1u;
+}
+// smod = s % 255; returns s - arrival in [0, 254]
+NC_HD int32_t nc_stamp_offset(int32_t smod, uint32_t code) {
+  int32_t o = smod - (int32_t)code + 1;
+  return o < 0 ? o + 255 : o;
+}
+
+// ---- per-slot record (16 B, one per incoming synapse) --------------------
+typedef struct __attribute__((aligned(16))) nc_slot {
+  float   w;        // Synapse::weight
+  float   depol;    // Synapse::AP_depolFac latched at fire time
+  int32_t last_arr; // Synapse::lastSpikeArrival as a step (NC_NEVER = -inf)
+  uint32_t flags;   // bit 0: inhibitory (sign class of the initial weight)
+} nc_slot;
+
+// ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers:
+// as easy as 1, 2, 3", SC'11).  Counter-based, so every draw is a pure
+// function of (key, counter) and is identical on host and device, in any
+// launch order or network packing.
+typedef struct { uint32_t v[4]; } nc_u32x4;
+
+NC_HD void nc_mulhilo32(uint32_t a, uint32_t b, uint32_t* hi, uint32_t* lo) {
+  uint64_t p = (uint64_t)a * (uint64_t)b;
+  *hi = (uint32_t)(p >> 32);
+  *lo = (uint32_t)p;
+}
+
+NC_HD nc_u32x4 nc_philox4x32_10(nc_u32x4 ctr, uint32_t k0, uint32_t k1) {
+  for (int r = 0; r < 10; ++r) {
+    uint32_t hi0, lo0, hi1, lo1;
+    nc_mulhilo32(0xD2511F53u, ctr.v[0], &hi0, &lo0);
+    nc_mulhilo32(0xCD9E8D57u, ctr.v[2], &hi1, &lo1);
+    nc_u32x4 o;
+    o.v[0] = hi1 ^ ctr.v[1] ^ k0;
+    o.v[1] = lo1;
+    o.v[2] = hi0 ^ ctr.v[3] ^ k1;
+    o.v[3] = lo0;
+    ctr = o;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+  return ctr;
+}
+
+// Draw streams (third counter word).
+#define NC_STREAM_POS    1u
+#define NC_STREAM_WEIGHT 2u
+#define NC_STREAM_INSEL  3u
+#define NC_STREAM_DRIVE  4u
+#define NC_STREAM_BG     5u
+
+NC_HD nc_u32x4 nc_draw(uint64_t seed, uint32_t a, uint32_t b, uint32_t stream) {
+  nc_u32x4 c;
+  c.v[0] = a; c.v[1] = b; c.v[2] = stream; c.v[3] = 0u;
+  return nc_philox4x32_10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+}
+
+// Uniform in [0,1) with 24 random bits: exact in float.
+NC_HD float nc_u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }

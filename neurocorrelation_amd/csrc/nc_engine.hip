@@ -1,0 +1,1144 @@
+// Host side of the MI355X engine: device-resident pools of networks, their
+// SELL-C-64 layout, the step tables and the C ABI of include/neucor_amd.h.
+//
+// Layout of one network in HBM (all structure-of-arrays):
+//   neurons are renumbered into "positions": windows of 1024 ids are sorted
+//   by in-degree (SELL-C-sigma, C = 64, sigma = 1024) and cut into slices of
+//   64 lanes, one wave per slice.  The incoming synapses of a slice are
+//   stored chunk-major: chunk c of lane l holds the lane's slots 4c..4c+3 at
+//   slot index slice_off + c*256 + l*4, so a wave reads 256 contiguous stamp
+//   bytes per chunk.  Slot order inside a lane is ascending source id.
+//   stamp  : u8 per slot (spike latch + arrival step code)
+//   slots  : 16 B per slot {w, depol, last_arr, flags}
+//   out-CSR: per position a row of slot indices (+ u8 delays) of the
+//            synapses it feeds, for the fan-out.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "nc_common.h"
+#include "nc_internal.h"
+#include "nc_kernels.h"
+
+using nc::fail;
+
+namespace {
+
+#define NC_HIP(call)                                                              \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess)                                                         \
+      return fail(e_ == hipErrorOutOfMemory ? NC_ERR_OOM : NC_ERR_HIP,            \
+                  std::string(#call) + ": " + hipGetErrorString(e_));             \
+  } while (0)
+
+constexpr uint32_t kSigma = 1024;
+
+// First-fit allocator over [0, cap) in units.
+struct RangeAlloc {
+  uint64_t cap = 0;
+  std::map<uint64_t, uint64_t> free_;  // offset -> length
+  void init(uint64_t c) {
+    cap = c;
+    free_.clear();
+    if (c) free_[0] = c;
+  }
+  bool alloc(uint64_t n, uint64_t* off) {
+    if (n == 0) { *off = 0; return true; }
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= n) {
+        *off = it->first;
+        uint64_t rest = it->second - n, o = it->first + n;
+        free_.erase(it);
+        if (rest) free_[o] = rest;
+        return true;
+      }
+    }
+    return false;
+  }
+  void release(uint64_t off, uint64_t n) {
+    if (n == 0) return;
+    auto it = free_.emplace(off, n).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+      it->second += nx->second;
+      free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+      auto pv = std::prev(it);
+      if (pv->first + pv->second == it->first) {
+        pv->second += it->second;
+        free_.erase(it);
+      }
+    }
+  }
+  uint64_t free_units() const {
+    uint64_t f = 0;
+    for (auto& kv : free_) f += kv.second;
+    return f;
+  }
+};
+
+struct Layout {
+  uint32_t n = 0, nq = 0;
+  std::vector<uint32_t> pos_of_id, id_of_pos;  // id_of_pos: nq*64 (~0u = padding)
+  std::vector<uint32_t> slice_slots;           // slots per slice (multiple of 256)
+  uint64_t nslots = 0;
+};
+
+void make_layout(const nc_net& net, Layout* L) {
+  const uint32_t n = net.n;
+  L->n = n;
+  L->nq = (n + 63) / 64;
+  L->pos_of_id.resize(n);
+  L->id_of_pos.assign((size_t)L->nq * 64, ~0u);
+  std::vector<uint32_t> ids(n);
+  std::iota(ids.begin(), ids.end(), 0u);
+  for (uint32_t w0 = 0; w0 < n; w0 += kSigma) {
+    uint32_t w1 = std::min(n, w0 + kSigma);
+    std::stable_sort(ids.begin() + w0, ids.begin() + w1, [&](uint32_t a, uint32_t b) {
+      return (net.in_ptr[a + 1] - net.in_ptr[a]) > (net.in_ptr[b + 1] - net.in_ptr[b]);
+    });
+  }
+  for (uint32_t p = 0; p < n; ++p) {
+    L->id_of_pos[p] = ids[p];
+    L->pos_of_id[ids[p]] = p;
+  }
+  L->slice_slots.resize(L->nq);
+  L->nslots = 0;
+  for (uint32_t q = 0; q < L->nq; ++q) {
+    uint32_t mx = 0;
+    for (uint32_t l = 0; l < 64; ++l) {
+      uint32_t id = L->id_of_pos[q * 64 + l];
+      if (id == ~0u) continue;
+      uint32_t d = (uint32_t)(net.in_ptr[id + 1] - net.in_ptr[id]);
+      mx = std::max(mx, d);
+    }
+    uint32_t nch = (mx + 3) / 4;
+    L->slice_slots[q] = nch * 256;
+    L->nslots += L->slice_slots[q];
+  }
+}
+
+struct Tables {
+  float dt = 0.f, pre_decay = 0.f, post_decay = 0.f;
+  std::vector<float> t_pre, t_post, w_ap;
+  int32_t d_ref = 0, r_rel = 0;
+  float P = 0.f, depol_k = 0.f;
+  double e = 0.0;
+};
+
+// Every transcendental the reference evaluates in a step, as a function of
+// an integer step difference d, evaluated with this host's libm exactly as
+// the reference writes it.
+void build_tables(float dt, float pre_decay, float post_decay, Tables* T) {
+  T->dt = dt;
+  T->pre_decay = pre_decay;
+  T->post_decay = post_decay;
+  // Synapse trace powf(presynapticTraceDecay, t - lastSpikeArrival) and the
+  // neuron trace powf(postsynapticTraceDecay, t - lastFire), with the
+  // "== 1 -> 0" rule (src/NeuCor.cpp:741-745, :671-675).
+  auto trace_table = [dt](float decay, std::vector<float>* t) {
+    t->clear();
+    for (int32_t d = 0;; ++d) {
+      float x = (float)d * dt;
+      float v = powf(decay, x);
+      if (v == 1.0f) v = 0.0f;
+      if (v == 0.0f && d > 0) break;
+      t->push_back(v);
+      if (d > 100000000) break;
+    }
+  };
+  trace_table(pre_decay, &T->t_pre);
+  trace_table(post_decay, &T->t_post);
+  // Refractory / AP window: AP_cutoff < t - lastFire (src/NeuCor.cpp:684, :707).
+  int32_t dref = 0;
+  while (!(NC_AP_CUTOFF < (float)(dref + 1) * dt)) ++dref;
+  T->d_ref = dref;
+  // Latch release: the contribution at offset o is added, then the latch
+  // clears when AP_cutoff < o*dt (src/NeuCor.cpp:692-697).
+  T->r_rel = dref + 1;
+  // AP waveform (src/NeuCor.cpp:706-714) with lastFire = 0, currentT = d*dt.
+  T->w_ap.resize(dref + 1);
+  for (int32_t d = 0; d <= dref; ++d) {
+    const float lastFire = 0.0f;
+    const float currentT = (float)d * dt;
+    float currentAP =
+        NC_AP_H * (exp(-powf((currentT - lastFire) - NC_AP_DSTART, 2.0) /
+                       (2.0 * NC_AP_DEPOLW * NC_AP_DEPOLW)) -
+                   exp(-powf((currentT - lastFire) - NC_AP_DSTART - NC_AP_DELTAPOL, 2.0) /
+                       (2.0 * NC_AP_POLW * NC_AP_POLW)) *
+                       NC_AP_DEPOLFAC) +
+        NC_BASELEVEL + (NC_THRESHOLD - NC_BASELEVEL) * fmax(1.0 + lastFire - currentT, 0.0);
+    T->w_ap[d] = currentAP;
+  }
+  T->P = powf(NC_RECHARGE, dt);         // powf(recharge, deltaT) (:678)
+  T->e = exp(NC_CHARGE_R * (double)dt);  // exp(0.3702*deltaT) (:695)
+  float dfac = NC_AP_DEPOLFAC;
+  dfac *= NC_SYN_DEPOL_MUL;  // AP_depolFac *= 52.0 (:730)
+  T->depol_k = dfac;
+}
+
+struct NetRec {
+  bool live = false;
+  uint32_t n = 0;
+  uint64_t S = 0;
+  uint32_t q0 = 0, nq = 0;
+  uint64_t slot0 = 0, nslots = 0, out0 = 0;
+  std::vector<uint32_t> pos_of_id;    // global positions
+  std::vector<uint32_t> slot_of_syn;  // global slot per synapse, export order
+  std::vector<uint32_t> id_of_pos;    // local, nq*64
+  float lr = 1.f, pre_f = 0.13f, post_f = 0.30f;
+  uint32_t steps_done = 0, steps_left = UINT32_MAX;
+  uint32_t grp0 = 0, ngrp = 0, thr0 = 0, nthr = 0;
+  nc_drive_spec drive{};
+  bool has_drive = false;
+  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint8_t>>> events;  // abs step, (pos, kind)
+};
+
+template <class T>
+struct DevBuf {
+  T* p = nullptr;
+  size_t n = 0;
+};
+
+}  // namespace
+
+struct nc_sim {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  uint64_t max_pos = 0, max_slots = 0, max_out = 0;
+  uint32_t max_nets = 0, raster_cap = 0;
+  RangeAlloc a_slices, a_slots, a_out;
+  std::vector<NetRec> nets;
+  bool tables_set = false;
+  Tables T;
+  uint64_t gstep = 0;
+  int kernel_timing = 0;  // bit mask of timed kernels
+  uint64_t k_launch[4] = {0, 0, 0, 0};
+  double k_ms[4] = {0, 0, 0, 0};
+  std::vector<hipEvent_t> ev_pool;
+
+  // device buffers
+  uint8_t* stamp = nullptr;
+  nc_slot* slots = nullptr;
+  float* V = nullptr;
+  int32_t* last_fire = nullptr;
+  uint32_t *firings = nullptr, *degw = nullptr, *nid = nullptr, *in_group = nullptr;
+  uint32_t *out_beg = nullptr, *out_cnt = nullptr, *out_slot = nullptr;
+  uint8_t* out_delay = nullptr;
+  uint32_t *slice_off = nullptr, *slice_net = nullptr, *active = nullptr;
+  uint32_t *forced_in = nullptr, *forced_bg = nullptr;
+  uint32_t *fired = nullptr, *fired_cnt = nullptr;
+  uint32_t *raster = nullptr, *raster_cnt = nullptr;
+  unsigned long long* ctr = nullptr;
+  nc_netdesc* d_nets = nullptr;
+  float *t_pre = nullptr, *t_post = nullptr, *w_ap = nullptr;
+  DevBuf<nc_group> d_groups;
+  DevBuf<uint32_t> d_thr;
+  DevBuf<uint32_t> d_ev_pos;
+  DevBuf<uint8_t> d_ev_kind;
+
+  std::vector<nc_group> groups;
+  std::vector<uint32_t> thr;
+  std::vector<uint32_t> active_host;
+  bool active_dirty = true, groups_dirty = true;
+  std::vector<uint32_t> slice_net_host;
+  uint64_t raster_read = 0;
+  // Host staging of the per-call uploads; reused only after upload_done.
+  std::vector<nc_netdesc> h_nds;
+  std::vector<uint32_t> h_ev_pos;
+  std::vector<uint8_t> h_ev_kind;
+  hipEvent_t upload_done = nullptr;
+  bool upload_pending = false;
+};
+
+namespace {
+
+nc_status check_net(nc_sim* sim, int32_t id, NetRec** out) {
+  if (!sim) return fail(NC_ERR_ARG, "null sim");
+  if (id < 0 || (size_t)id >= sim->nets.size() || !sim->nets[id].live)
+    return fail(NC_ERR_RANGE, "no such network id");
+  *out = &sim->nets[id];
+  return NC_OK;
+}
+
+template <class T>
+nc_status grow(DevBuf<T>* b, size_t n, hipStream_t st) {
+  if (n <= b->n && b->p) return NC_OK;
+  size_t cap = std::max<size_t>(n, b->n * 2);
+  if (cap < 64) cap = 64;
+  NC_HIP(hipStreamSynchronize(st));
+  if (b->p) NC_HIP(hipFree(b->p));
+  b->p = nullptr;
+  b->n = 0;
+  NC_HIP(hipMalloc(&b->p, cap * sizeof(T)));
+  b->n = cap;
+  return NC_OK;
+}
+
+nc_step_args base_args(nc_sim* sim) {
+  nc_step_args a{};
+  a.stamp = sim->stamp;
+  a.slots = sim->slots;
+  a.V = sim->V;
+  a.last_fire = sim->last_fire;
+  a.firings = sim->firings;
+  a.degw = sim->degw;
+  a.nid = sim->nid;
+  a.in_group = sim->in_group;
+  a.slice_off = sim->slice_off;
+  a.slice_net = sim->slice_net;
+  a.active_slices = sim->active;
+  a.n_active_slices = (uint32_t)sim->active_host.size();
+  a.nets = sim->d_nets;
+  a.groups = sim->d_groups.p;
+  a.thr = sim->d_thr.p;
+  a.forced_in = sim->forced_in;
+  a.forced_bg = sim->forced_bg;
+  a.raster = sim->raster;
+  a.raster_cnt = sim->raster_cnt;
+  a.raster_cap = sim->raster_cap;
+  a.ctr = sim->ctr;
+  a.t_pre = sim->t_pre;
+  a.t_post = sim->t_post;
+  a.w_ap = sim->w_ap;
+  a.len_pre = (int32_t)sim->T.t_pre.size();
+  a.len_post = (int32_t)sim->T.t_post.size();
+  a.dt = sim->T.dt;
+  a.P = sim->T.P;
+  a.depol_k = sim->T.depol_k;
+  a.c0 = NC_CHARGE_K;
+  a.e = sim->T.e;
+  a.d_ref = sim->T.d_ref;
+  a.r_rel = sim->T.r_rel;
+  a.out_beg = sim->out_beg;
+  a.out_cnt = sim->out_cnt;
+  a.out_slot = sim->out_slot;
+  a.out_delay = sim->out_delay;
+  return a;
+}
+
+// Pending arrivals of the last completed step get their STDP now.
+nc_status flush_net(nc_sim* sim, NetRec& r) {
+  if (r.steps_done == 0 || r.nq == 0) return NC_OK;
+  nc_step_args a = base_args(sim);
+  uint32_t blocks = (r.nq + 3) / 4;
+  hipLaunchKernelGGL(k_flush, dim3(blocks), dim3(256), 0, sim->stream, a, r.q0, r.nq,
+                     (int32_t)r.steps_done, r.lr, r.pre_f, r.post_f);
+  NC_HIP(hipGetLastError());
+  return NC_OK;
+}
+
+hipEvent_t get_event(nc_sim* sim, size_t i) {
+  while (sim->ev_pool.size() <= i) {
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    sim->ev_pool.push_back(e);
+  }
+  return sim->ev_pool[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64_t* bytes) {
+  if (!bytes) return fail(NC_ERR_ARG, "null argument");
+  uint64_t npos = (n_neurons + 63) / 64 * 64;
+  uint64_t slots = n_syn + n_syn / 4 + npos * 4;  // padding allowance
+  *bytes = npos * 48 + slots * 17 + n_syn * 5;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, uint64_t* slots) {
+  if (!net) return fail(NC_ERR_ARG, "null network");
+  Layout L;
+  make_layout(*net, &L);
+  if (positions) *positions = (uint64_t)L.nq * 64;
+  if (slots) *slots = L.nslots;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
+  if (!spec || !out) return fail(NC_ERR_ARG, "null argument");
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return fail(NC_ERR_NO_DEVICE, "no HIP device");
+  if (spec->device < 0 || spec->device >= ndev) return fail(NC_ERR_RANGE, "device index out of range");
+  if (spec->max_neurons == 0 || spec->max_nets == 0) return fail(NC_ERR_ARG, "zero capacity");
+  if (spec->max_synapses >= (1ull << 32) - (1ull << 24)) return fail(NC_ERR_ARG, "max_synapses too large");
+  nc_sim* sim = new (std::nothrow) nc_sim();
+  if (!sim) return fail(NC_ERR_OOM, "host allocation failed");
+  sim->device = spec->device;
+  nc_status st = NC_OK;
+  auto H = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && st == NC_OK)
+      st = fail(e == hipErrorOutOfMemory ? NC_ERR_OOM : NC_ERR_HIP,
+                std::string(what) + ": " + hipGetErrorString(e));
+  };
+  H(hipSetDevice(spec->device), "hipSetDevice");
+  H(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking), "hipStreamCreate");
+  H(hipEventCreateWithFlags(&sim->upload_done, hipEventDisableTiming), "hipEventCreate");
+  sim->max_pos = (spec->max_neurons + 63) / 64 * 64 + 64 * (uint64_t)spec->max_nets;
+  sim->max_slots = spec->max_slots
+                       ? (spec->max_slots + 255) / 256 * 256
+                       : (spec->max_synapses + spec->max_synapses / 4 + sim->max_pos * 4 + 255) / 256 * 256;
+  if (sim->max_slots >= (1ull << 32)) sim->max_slots = ((1ull << 32) - 256);
+  sim->max_out = spec->max_synapses + 64;
+  sim->max_nets = spec->max_nets;
+  sim->raster_cap = spec->raster_capacity;
+  const uint64_t P = sim->max_pos, Q = P / 64;
+  if (st == NC_OK) {
+    H(hipMalloc(&sim->stamp, sim->max_slots), "hipMalloc stamp");
+    H(hipMalloc(&sim->slots, sim->max_slots * sizeof(nc_slot)), "hipMalloc slots");
+    H(hipMalloc(&sim->V, P * 4), "hipMalloc V");
+    H(hipMalloc(&sim->last_fire, P * 4), "hipMalloc lastFire");
+    H(hipMalloc(&sim->firings, P * 4), "hipMalloc firings");
+    H(hipMalloc(&sim->degw, P * 4), "hipMalloc degw");
+    H(hipMalloc(&sim->nid, P * 4), "hipMalloc nid");
+    H(hipMalloc(&sim->in_group, P * 4), "hipMalloc in_group");
+    H(hipMalloc(&sim->out_beg, P * 4), "hipMalloc out_beg");
+    H(hipMalloc(&sim->out_cnt, P * 4), "hipMalloc out_cnt");
+    H(hipMalloc(&sim->out_slot, sim->max_out * 4), "hipMalloc out_slot");
+    H(hipMalloc(&sim->out_delay, sim->max_out), "hipMalloc out_delay");
+    H(hipMalloc(&sim->slice_off, Q * 4), "hipMalloc slice_off");
+    H(hipMalloc(&sim->slice_net, Q * 4), "hipMalloc slice_net");
+    H(hipMalloc(&sim->active, Q * 4), "hipMalloc active");
+    H(hipMalloc(&sim->forced_in, (P / 32) * 4), "hipMalloc forced_in");
+    H(hipMalloc(&sim->forced_bg, (P / 32) * 4), "hipMalloc forced_bg");
+    H(hipMalloc(&sim->fired, 2 * P * 4), "hipMalloc fired");
+    H(hipMalloc(&sim->fired_cnt, 64), "hipMalloc fired_cnt");
+    H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
+    if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
+    H(hipMalloc(&sim->ctr, NC_CTR_ROWS * NC_NCTR * 8), "hipMalloc ctr");
+    H(hipMalloc(&sim->d_nets, sim->max_nets * sizeof(nc_netdesc)), "hipMalloc nets");
+  }
+  if (st == NC_OK) {
+    H(hipMemsetAsync(sim->stamp, 0, sim->max_slots, sim->stream), "memset");
+    H(hipMemsetAsync(sim->degw, 0, P * 4, sim->stream), "memset");
+    H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
+    H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
+    H(hipMemsetAsync(sim->fired_cnt, 0, 64, sim->stream), "memset");
+    H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
+    H(hipMemsetAsync(sim->ctr, 0, NC_CTR_ROWS * NC_NCTR * 8, sim->stream), "memset");
+    std::vector<uint32_t> none(Q, NC_SLICE_NONE);
+    sim->slice_net_host = none;
+    H(hipMemcpyAsync(sim->slice_net, none.data(), Q * 4, hipMemcpyHostToDevice, sim->stream), "memcpy");
+    H(hipStreamSynchronize(sim->stream), "sync");
+  }
+  sim->a_slices.init(Q);
+  sim->a_slots.init(sim->max_slots / 256);
+  sim->a_out.init(sim->max_out);
+  if (st != NC_OK) {
+    nc_sim_destroy(sim);
+    return st;
+  }
+  *out = sim;
+  return NC_OK;
+}
+
+extern "C" void nc_sim_destroy(nc_sim* sim) {
+  if (!sim) return;
+  (void)hipSetDevice(sim->device);
+  if (sim->stream) (void)hipStreamSynchronize(sim->stream);
+  void* ptrs[] = {sim->stamp, sim->slots, sim->V, sim->last_fire, sim->firings, sim->degw,
+                  sim->nid, sim->in_group, sim->out_beg, sim->out_cnt, sim->out_slot,
+                  sim->out_delay, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
+                  sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
+                  sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p};
+  for (void* p : ptrs)
+    if (p) (void)hipFree(p);
+  for (hipEvent_t e : sim->ev_pool) (void)hipEventDestroy(e);
+  if (sim->upload_done) (void)hipEventDestroy(sim->upload_done);
+  if (sim->stream) (void)hipStreamDestroy(sim->stream);
+  delete sim;
+}
+
+extern "C" nc_status nc_sim_free_bytes(const nc_sim* sim, uint64_t* free_bytes) {
+  if (!sim || !free_bytes) return fail(NC_ERR_ARG, "null argument");
+  NC_HIP(hipSetDevice(sim->device));
+  size_t fr = 0, tot = 0;
+  NC_HIP(hipMemGetInfo(&fr, &tot));
+  *free_bytes = fr;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params* params,
+                                const nc_drive_spec* drive, uint32_t steps_to_run,
+                                int32_t* net_id) {
+  if (!sim || !net || !params || !net_id) return fail(NC_ERR_ARG, "null argument");
+  NC_HIP(hipSetDevice(sim->device));
+  // The drive tables below are host vectors a pending upload may still read.
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  const nc_params p = *params;
+  {
+    // dt must be 2^-m, 0 <= m <= 8, so that float time is exact on the grid.
+    int ex = 0;
+    float fr = frexpf(p.dt, &ex);
+    if (!(p.dt > 0.0f) || fr != 0.5f || ex > 1 || ex < -7)
+      return fail(NC_ERR_ARG, "dt must be a power of two in [2^-8, 1] ms");
+  }
+  if (!sim->tables_set) {
+    build_tables(p.dt, p.pre_trace_decay, p.post_trace_decay, &sim->T);
+    const Tables& T = sim->T;
+    NC_HIP(hipMalloc(&sim->t_pre, std::max<size_t>(1, T.t_pre.size()) * 4));
+    NC_HIP(hipMalloc(&sim->t_post, std::max<size_t>(1, T.t_post.size()) * 4));
+    NC_HIP(hipMalloc(&sim->w_ap, T.w_ap.size() * 4));
+    NC_HIP(hipMemcpy(sim->t_pre, T.t_pre.data(), T.t_pre.size() * 4, hipMemcpyHostToDevice));
+    NC_HIP(hipMemcpy(sim->t_post, T.t_post.data(), T.t_post.size() * 4, hipMemcpyHostToDevice));
+    NC_HIP(hipMemcpy(sim->w_ap, T.w_ap.data(), T.w_ap.size() * 4, hipMemcpyHostToDevice));
+    sim->tables_set = true;
+  } else if (p.dt != sim->T.dt || p.pre_trace_decay != sim->T.pre_decay ||
+             p.post_trace_decay != sim->T.post_decay) {
+    return fail(NC_ERR_ARG, "all networks of one nc_sim share dt and the trace decays");
+  }
+  if ((uint32_t)net->max_delay + (uint32_t)sim->T.r_rel >= 254u)
+    return fail(NC_ERR_ARG, "max delay too long for this dt (delay + release window must be < 254 steps)");
+  if (net->max_in_degree > NC_DEG_MASK) return fail(NC_ERR_ARG, "in-degree too large");
+
+  // Free network id.
+  int32_t id = -1;
+  for (size_t i = 0; i < sim->nets.size(); ++i)
+    if (!sim->nets[i].live) { id = (int32_t)i; break; }
+  if (id < 0) {
+    if (sim->nets.size() >= sim->max_nets) return fail(NC_ERR_CAPACITY, "network table full");
+    sim->nets.emplace_back();
+    id = (int32_t)sim->nets.size() - 1;
+  }
+
+  Layout L;
+  make_layout(*net, &L);
+  const uint64_t S = net->n_syn();
+  uint64_t q0 = 0, sl0 = 0, o0 = 0;
+  if (!sim->a_slices.alloc(L.nq, &q0)) return fail(NC_ERR_CAPACITY, "no room for the neurons");
+  if (!sim->a_slots.alloc(L.nslots / 256, &sl0)) {
+    sim->a_slices.release(q0, L.nq);
+    return fail(NC_ERR_CAPACITY, "no room for the synapses");
+  }
+  if (!sim->a_out.alloc(S, &o0)) {
+    sim->a_slices.release(q0, L.nq);
+    sim->a_slots.release(sl0, L.nslots / 256);
+    return fail(NC_ERR_CAPACITY, "no room for the out-synapse lists");
+  }
+  sl0 *= 256;
+
+  NetRec& r = sim->nets[id];
+  r = NetRec();
+  r.n = net->n;
+  r.S = S;
+  r.q0 = (uint32_t)q0;
+  r.nq = L.nq;
+  r.slot0 = sl0;
+  r.nslots = L.nslots;
+  r.out0 = o0;
+  r.lr = p.learning_rate;
+  r.pre_f = p.pre_factor;
+  r.post_f = p.post_factor;
+  r.steps_left = steps_to_run ? steps_to_run : UINT32_MAX;
+  r.id_of_pos = L.id_of_pos;
+  r.pos_of_id.resize(net->n);
+  for (uint32_t i = 0; i < net->n; ++i) r.pos_of_id[i] = (uint32_t)(q0 * 64) + L.pos_of_id[i];
+
+  const uint32_t npos = L.nq * 64;
+  const uint32_t pos0 = (uint32_t)(q0 * 64);
+  std::vector<uint32_t> slice_off(L.nq);
+  {
+    uint64_t o = sl0;
+    for (uint32_t q = 0; q < L.nq; ++q) {
+      slice_off[q] = (uint32_t)o;
+      o += L.slice_slots[q];
+    }
+  }
+  // Slot records, in layout order.
+  std::vector<nc_slot> hs(L.nslots);
+  for (auto& x : hs) x = nc_slot{0.0f, 0.0f, NC_NEVER, 0u};
+  r.slot_of_syn.resize(S);
+  for (uint32_t id_ = 0; id_ < net->n; ++id_) {
+    const uint32_t pl = L.pos_of_id[id_];
+    const uint32_t q = pl / 64, l = pl % 64;
+    const int64_t b = net->in_ptr[id_], e = net->in_ptr[id_ + 1];
+    for (int64_t k = 0; k < e - b; ++k) {
+      uint32_t j = slice_off[q] + (uint32_t)((k >> 2) * 256 + l * 4 + (k & 3));
+      r.slot_of_syn[b + k] = j;
+      nc_slot& x = hs[j - sl0];
+      x.w = net->w[b + k];
+      x.flags = net->inh[b + k] ? 1u : 0u;
+    }
+  }
+  // Out-CSR by source position; rows in ascending target id.
+  std::vector<uint32_t> outcnt(npos, 0), outbeg(npos, 0);
+  for (uint64_t k = 0; k < S; ++k) outcnt[L.pos_of_id[net->in_src[k]]]++;
+  {
+    uint64_t o = o0;
+    for (uint32_t pl = 0; pl < npos; ++pl) {
+      outbeg[pl] = (uint32_t)o;
+      o += outcnt[pl];
+    }
+  }
+  std::vector<uint32_t> oslot(S);
+  std::vector<uint8_t> odly(S);
+  {
+    std::vector<uint32_t> fillp(outbeg);
+    for (uint32_t tg = 0; tg < net->n; ++tg) {
+      for (int64_t k = net->in_ptr[tg]; k < net->in_ptr[tg + 1]; ++k) {
+        uint32_t sp = L.pos_of_id[net->in_src[k]];
+        uint32_t e = fillp[sp]++ - (uint32_t)o0;
+        oslot[e] = r.slot_of_syn[k];
+        odly[e] = net->delay[k];
+      }
+    }
+  }
+  // Drive.
+  std::vector<uint32_t> ingrp(npos, 0), degw(npos, 0), nidv(npos, ~0u);
+  for (uint32_t pl = 0; pl < npos; ++pl) {
+    uint32_t idl = L.id_of_pos[pl];
+    if (idl == ~0u) continue;
+    degw[pl] = (uint32_t)(net->in_ptr[idl + 1] - net->in_ptr[idl]) | NC_DEG_VALID;
+    nidv[pl] = idl;
+  }
+  nc_netdesc nd{};
+  nd.lr = r.lr;
+  nd.pre_f = r.pre_f;
+  nd.post_f = r.post_f;
+  if (drive) {
+    nc::InputSel sel;
+    nc_status stt = nc::select_inputs(*net, *drive, &sel);
+    if (stt != NC_OK) {
+      sim->a_slices.release(q0, L.nq);
+      sim->a_slots.release(sl0 / 256, L.nslots / 256);
+      sim->a_out.release(o0, S);
+      r.live = false;
+      return stt;
+    }
+    r.has_drive = true;
+    r.drive = *drive;
+    r.grp0 = (uint32_t)sim->groups.size();
+    r.ngrp = sel.n_groups_total;
+    r.thr0 = (uint32_t)sim->thr.size();
+    const uint32_t T = drive->period ? drive->period : 1;
+    const uint32_t ncorr = drive->n_correlated > 0 ? drive->n_groups : 0;
+    for (uint32_t g = 0; g < sel.n_groups_total; ++g) {
+      nc_group G{};
+      G.tbl_off = (uint32_t)sim->thr.size();
+      G.period = T;
+      G.local = g;
+      const double amp = g < ncorr ? drive->amp_corr : drive->amp_ind;
+      for (uint32_t k = 0; k < T; ++k) {
+        double pr = (double)drive->p0 * (1.0 + amp * sin(2.0 * M_PI * (double)k / (double)T));
+        uint32_t th;
+        if (!(pr > 0.0)) th = 0;
+        else if (pr >= 1.0) th = 0xFFFFFFFFu;
+        else th = (uint32_t)floor(pr * 4294967296.0);
+        sim->thr.push_back(th);
+      }
+      sim->groups.push_back(G);
+    }
+    r.nthr = (uint32_t)sim->thr.size() - r.thr0;
+    for (size_t k = 0; k < sel.ids.size(); ++k) {
+      uint32_t pl = L.pos_of_id[sel.ids[k]];
+      degw[pl] |= NC_DEG_INPUT;
+      ingrp[pl] = r.grp0 + sel.group[k];
+    }
+    nd.drive_lo = (uint32_t)drive->seed;
+    nd.drive_hi = (uint32_t)(drive->seed >> 32);
+    if (drive->bg_prob > 0.0f) {
+      double th = floor((double)drive->bg_prob * 4294967296.0);
+      nd.bg_thr = th >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)th;
+      if (nd.bg_thr == 0) nd.bg_thr = 1;
+      uint64_t bs = drive->seed ^ 0x9E3779B97F4A7C15ull;
+      nd.bg_lo = (uint32_t)bs;
+      nd.bg_hi = (uint32_t)(bs >> 32);
+    }
+    sim->groups_dirty = true;
+  }
+  std::vector<float> V0(npos, NC_BASELEVEL);
+  std::vector<int32_t> lf0(npos, NC_NEVER);
+  std::vector<uint32_t> zero(npos, 0), snet(L.nq, (uint32_t)id);
+  hipStream_t st = sim->stream;
+  NC_HIP(hipMemcpyAsync(sim->slots + sl0, hs.data(), L.nslots * sizeof(nc_slot), hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemsetAsync(sim->stamp + sl0, 0, L.nslots, st));
+  NC_HIP(hipMemcpyAsync(sim->V + pos0, V0.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf0.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->firings + pos0, zero.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->degw + pos0, degw.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->nid + pos0, nidv.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->in_group + pos0, ingrp.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->out_beg + pos0, outbeg.data(), npos * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->out_cnt + pos0, outcnt.data(), npos * 4, hipMemcpyHostToDevice, st));
+  if (S) {
+    NC_HIP(hipMemcpyAsync(sim->out_slot + o0, oslot.data(), S * 4, hipMemcpyHostToDevice, st));
+    NC_HIP(hipMemcpyAsync(sim->out_delay + o0, odly.data(), S, hipMemcpyHostToDevice, st));
+  }
+  NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), L.nq * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), L.nq * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
+  NC_HIP(hipStreamSynchronize(st));
+  for (uint32_t q = 0; q < L.nq; ++q) sim->slice_net_host[q0 + q] = (uint32_t)id;
+  r.live = true;
+  sim->active_dirty = true;
+  *net_id = id;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) {
+  NetRec* r;
+  nc_status stt = check_net(sim, net_id, &r);
+  if (stt != NC_OK) return stt;
+  NC_HIP(hipSetDevice(sim->device));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  std::vector<uint32_t> none(r->nq, NC_SLICE_NONE);
+  NC_HIP(hipMemcpy(sim->slice_net + r->q0, none.data(), r->nq * 4, hipMemcpyHostToDevice));
+  for (uint32_t q = 0; q < r->nq; ++q) sim->slice_net_host[r->q0 + q] = NC_SLICE_NONE;
+  sim->a_slices.release(r->q0, r->nq);
+  sim->a_slots.release(r->slot0 / 256, r->nslots / 256);
+  sim->a_out.release(r->out0, r->S);
+  // Drive groups stay allocated in the host vectors (append-only); they are
+  // compacted when no network uses them.
+  r->live = false;
+  r->pos_of_id.clear();
+  r->slot_of_syn.clear();
+  sim->active_dirty = true;
+  bool any = false;
+  for (auto& x : sim->nets) any |= x.live;
+  if (!any) {
+    sim->groups.clear();
+    sim->thr.clear();
+    sim->groups_dirty = true;
+  }
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
+  if (!sim) return fail(NC_ERR_ARG, "null sim");
+  if (n_steps == 0) return NC_OK;
+  NC_HIP(hipSetDevice(sim->device));
+  hipStream_t st = sim->stream;
+  if (sim->groups_dirty) {
+    if (!sim->groups.empty()) {
+      nc_status s1 = grow(&sim->d_groups, sim->groups.size(), st);
+      if (s1 != NC_OK) return s1;
+      s1 = grow(&sim->d_thr, sim->thr.size(), st);
+      if (s1 != NC_OK) return s1;
+      NC_HIP(hipMemcpyAsync(sim->d_groups.p, sim->groups.data(), sim->groups.size() * sizeof(nc_group),
+                            hipMemcpyHostToDevice, st));
+      NC_HIP(hipMemcpyAsync(sim->d_thr.p, sim->thr.data(), sim->thr.size() * 4, hipMemcpyHostToDevice, st));
+    }
+    sim->groups_dirty = false;
+  }
+  // Per-network window of this call.
+  if (sim->upload_pending) {
+    NC_HIP(hipEventSynchronize(sim->upload_done));
+    sim->upload_pending = false;
+  }
+  std::vector<nc_netdesc>& nds = sim->h_nds;
+  nds.assign(sim->nets.size(), nc_netdesc{});
+  bool any_active = false;
+  uint32_t max_run = 0;
+  for (size_t i = 0; i < sim->nets.size(); ++i) {
+    NetRec& r = sim->nets[i];
+    nc_netdesc& nd = nds[i];
+    nd = nc_netdesc{};
+    if (!r.live) continue;
+    uint32_t run = std::min<uint64_t>(n_steps, r.steps_left);
+    if ((uint64_t)r.steps_done + run >= (uint64_t)NC_MAX_STEPS)
+      return fail(NC_ERR_RANGE, "step count limit reached");
+    nd.base_step = (int32_t)r.steps_done;
+    nd.remaining = (int32_t)run;
+    nd.lr = r.lr;
+    nd.pre_f = r.pre_f;
+    nd.post_f = r.post_f;
+    if (r.has_drive) {
+      nd.drive_lo = (uint32_t)r.drive.seed;
+      nd.drive_hi = (uint32_t)(r.drive.seed >> 32);
+      if (r.drive.bg_prob > 0.0f) {
+        double th = floor((double)r.drive.bg_prob * 4294967296.0);
+        nd.bg_thr = th >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)th;
+        if (nd.bg_thr == 0) nd.bg_thr = 1;
+        uint64_t bs = r.drive.seed ^ 0x9E3779B97F4A7C15ull;
+        nd.bg_lo = (uint32_t)bs;
+        nd.bg_hi = (uint32_t)(bs >> 32);
+      }
+    }
+    if (run) any_active = true;
+    max_run = std::max(max_run, run);
+  }
+  if (!any_active) return NC_OK;
+  if (sim->active_dirty) {
+    sim->active_host.clear();
+    for (size_t i = 0; i < sim->nets.size(); ++i) {
+      const NetRec& r = sim->nets[i];
+      if (!r.live) continue;
+      for (uint32_t q = 0; q < r.nq; ++q) sim->active_host.push_back(r.q0 + q);
+    }
+    if (!sim->active_host.empty())
+      NC_HIP(hipMemcpyAsync(sim->active, sim->active_host.data(), sim->active_host.size() * 4,
+                            hipMemcpyHostToDevice, st));
+    sim->active_dirty = false;
+  }
+  NC_HIP(hipMemcpyAsync(sim->d_nets, nds.data(), nds.size() * sizeof(nc_netdesc), hipMemcpyHostToDevice, st));
+
+  // Host-scheduled events for steps [1, max_run] of this call, per step.
+  std::vector<uint32_t>& ev_pos = sim->h_ev_pos;
+  std::vector<uint8_t>& ev_kind = sim->h_ev_kind;
+  ev_pos.clear();
+  ev_kind.clear();
+  std::vector<uint32_t> ev_ptr(max_run + 1, 0);
+  {
+    std::vector<std::vector<std::pair<uint32_t, uint8_t>>> per(max_run);
+    for (size_t i = 0; i < sim->nets.size(); ++i) {
+      NetRec& r = sim->nets[i];
+      if (!r.live || r.events.empty()) continue;
+      std::vector<std::pair<uint32_t, std::pair<uint32_t, uint8_t>>> keep;
+      const uint32_t run = (uint32_t)nds[i].remaining;
+      for (auto& ev : r.events) {
+        uint32_t k = ev.first - r.steps_done;  // 1-based offset in this call
+        if (k >= 1 && k <= run) per[k - 1].push_back(ev.second);
+        else if (k > run) keep.push_back(ev);
+      }
+      r.events.swap(keep);
+    }
+    for (uint32_t k = 0; k < max_run; ++k) {
+      ev_ptr[k + 1] = ev_ptr[k] + (uint32_t)per[k].size();
+      for (auto& e : per[k]) {
+        ev_pos.push_back(e.first);
+        ev_kind.push_back(e.second);
+      }
+    }
+  }
+  if (!ev_pos.empty()) {
+    nc_status s1 = grow(&sim->d_ev_pos, ev_pos.size(), st);
+    if (s1 != NC_OK) return s1;
+    s1 = grow(&sim->d_ev_kind, ev_kind.size(), st);
+    if (s1 != NC_OK) return s1;
+    NC_HIP(hipMemcpyAsync(sim->d_ev_pos.p, ev_pos.data(), ev_pos.size() * 4, hipMemcpyHostToDevice, st));
+    NC_HIP(hipMemcpyAsync(sim->d_ev_kind.p, ev_kind.data(), ev_kind.size(), hipMemcpyHostToDevice, st));
+  }
+  NC_HIP(hipEventRecord(sim->upload_done, st));
+  sim->upload_pending = true;
+
+  nc_step_args a = base_args(sim);
+  a.groups = sim->d_groups.p;
+  a.thr = sim->d_thr.p;
+  const uint32_t blocks1 = (a.n_active_slices + 3) / 4;
+  const uint32_t blocks_w = 1024;
+  size_t nev_used = 0;
+  std::vector<int> timed_kinds;
+  for (uint32_t i = 0; i < max_run; ++i) {
+    const uint32_t b = (uint32_t)(sim->gstep & 1);
+    a.chunk_i = (int32_t)i;
+    a.fired = sim->fired + (uint64_t)b * sim->max_pos;
+    a.fired_cnt = sim->fired_cnt + b;
+    a.fired_cnt_next = sim->fired_cnt + (b ^ 1);
+    a.n_ev = ev_ptr[i + 1] - ev_ptr[i];
+    a.ev_pos = sim->d_ev_pos.p + ev_ptr[i];
+    a.ev_kind = sim->d_ev_kind.p + ev_ptr[i];
+    a.use_forced = ev_pos.empty() ? 0 : 1;
+    auto launch = [&](int kind, auto fn, uint32_t grid) -> nc_status {
+      if (grid == 0) return NC_OK;
+      hipEvent_t e0 = nullptr, e1 = nullptr;
+      const bool timed = (sim->kernel_timing >> kind) & 1;
+      if (timed) {
+        e0 = get_event(sim, nev_used++);
+        e1 = get_event(sim, nev_used++);
+        if (!e0 || !e1) return fail(NC_ERR_HIP, "hipEventCreate failed");
+        NC_HIP(hipEventRecord(e0, st));
+      }
+      hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, a);
+      NC_HIP(hipGetLastError());
+      if (timed) {
+        NC_HIP(hipEventRecord(e1, st));
+        sim->k_launch[kind]++;
+        timed_kinds.push_back(kind);
+      }
+      return NC_OK;
+    };
+    nc_status s1 = NC_OK;
+    if (a.n_ev) s1 = launch(0, k_drive, (a.n_ev + 255) / 256);
+    if (s1 == NC_OK) s1 = launch(1, k_neuron, blocks1);
+    if (s1 == NC_OK) s1 = launch(2, k_post, blocks_w);
+    if (s1 == NC_OK) s1 = launch(3, k_fanout, blocks_w);
+    if (s1 != NC_OK) return s1;
+    sim->gstep++;
+  }
+  if (!timed_kinds.empty()) {
+    NC_HIP(hipEventSynchronize(sim->ev_pool[nev_used - 1]));
+    for (size_t q = 0; q < timed_kinds.size(); ++q) {
+      float ms = 0.f;
+      NC_HIP(hipEventElapsedTime(&ms, sim->ev_pool[2 * q], sim->ev_pool[2 * q + 1]));
+      sim->k_ms[timed_kinds[q]] += ms;
+    }
+  }
+  for (size_t i = 0; i < sim->nets.size(); ++i) {
+    NetRec& r = sim->nets[i];
+    if (!r.live) continue;
+    uint32_t run = (uint32_t)nds[i].remaining;
+    r.steps_done += run;
+    if (r.steps_left != UINT32_MAX) r.steps_left -= run;
+  }
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_progress(nc_sim* sim, int32_t net_id, uint32_t* steps_done,
+                                     uint32_t* steps_left) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (steps_done) *steps_done = r->steps_done;
+  if (steps_left) *steps_left = r->steps_left;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_get_time(nc_sim* sim, int32_t net_id, float* time_ms) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!time_ms) return fail(NC_ERR_ARG, "null argument");
+  *time_ms = (float)r->steps_done * sim->T.dt;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_set_params(nc_sim* sim, int32_t net_id, float learning_rate,
+                                   float pre_factor, float post_factor) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  NC_HIP(hipSetDevice(sim->device));
+  st = flush_net(sim, *r);
+  if (st != NC_OK) return st;
+  r->lr = learning_rate;
+  r->pre_f = pre_factor;
+  r->post_f = post_factor;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_events,
+                                    const uint32_t* step_off, const uint32_t* ids,
+                                    const uint8_t* kind) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (n_events && (!step_off || !ids || !kind)) return fail(NC_ERR_ARG, "null argument");
+  for (uint32_t k = 0; k < n_events; ++k) {
+    if (ids[k] >= r->n) return fail(NC_ERR_RANGE, "neuron id out of range");
+    if (step_off[k] < 1) return fail(NC_ERR_ARG, "step offsets start at 1");
+    if (kind[k] > 1) return fail(NC_ERR_ARG, "kind must be 0 (input) or 1 (background)");
+  }
+  for (uint32_t k = 0; k < n_events; ++k)
+    r->events.push_back({r->steps_done + step_off[k], {r->pos_of_id[ids[k]], kind[k]}});
+  return NC_OK;
+}
+
+namespace {
+template <class T>
+nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t n) {
+  if (!out) return fail(NC_ERR_ARG, "null argument");
+  if (n > r.n) return fail(NC_ERR_RANGE, "more neurons requested than the network has");
+  std::vector<T> tmp((size_t)r.nq * 64);
+  NC_HIP(hipMemcpyAsync(tmp.data(), dev + (size_t)r.q0 * 64, tmp.size() * sizeof(T), hipMemcpyDeviceToHost,
+                        sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  const uint32_t p0 = r.q0 * 64;
+  for (uint32_t i = 0; i < n; ++i) out[i] = tmp[r.pos_of_id[i] - p0];
+  return NC_OK;
+}
+
+nc_status read_slots(nc_sim* sim, NetRec& r, std::vector<nc_slot>* hs, std::vector<uint8_t>* hst) {
+  NC_HIP(hipSetDevice(sim->device));
+  nc_status st = flush_net(sim, r);
+  if (st != NC_OK) return st;
+  hs->resize(r.nslots);
+  NC_HIP(hipMemcpyAsync(hs->data(), sim->slots + r.slot0, r.nslots * sizeof(nc_slot), hipMemcpyDeviceToHost,
+                        sim->stream));
+  if (hst) {
+    hst->resize(r.nslots);
+    NC_HIP(hipMemcpyAsync(hst->data(), sim->stamp + r.slot0, r.nslots, hipMemcpyDeviceToHost, sim->stream));
+  }
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return NC_OK;
+}
+}  // namespace
+
+extern "C" nc_status nc_read_potential(nc_sim* sim, int32_t net_id, float* out, uint32_t n) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  NC_HIP(hipSetDevice(sim->device));
+  return read_pos_array(sim, *r, sim->V, out, n);
+}
+
+extern "C" nc_status nc_read_last_fire(nc_sim* sim, int32_t net_id, int32_t* out, uint32_t n) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  NC_HIP(hipSetDevice(sim->device));
+  return read_pos_array(sim, *r, sim->last_fire, out, n);
+}
+
+extern "C" nc_status nc_read_firings(nc_sim* sim, int32_t net_id, uint32_t* out, uint32_t n) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  NC_HIP(hipSetDevice(sim->device));
+  return read_pos_array(sim, *r, sim->firings, out, n);
+}
+
+extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, uint64_t n) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!out) return fail(NC_ERR_ARG, "null argument");
+  if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
+  std::vector<nc_slot> hs;
+  st = read_slots(sim, *r, &hs, nullptr);
+  if (st != NC_OK) return st;
+  for (uint64_t k = 0; k < n; ++k) out[k] = hs[r->slot_of_syn[k] - r->slot0].w;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* w, uint64_t n) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!w) return fail(NC_ERR_ARG, "null argument");
+  if (n > r->S) return fail(NC_ERR_RANGE, "more synapses given than the network has");
+  std::vector<nc_slot> hs;
+  st = read_slots(sim, *r, &hs, nullptr);
+  if (st != NC_OK) return st;
+  // Synapse::setWeight also re-derives the sign class (src/NeuCor.cpp:576-579).
+  for (uint64_t k = 0; k < n; ++k) {
+    nc_slot& x = hs[r->slot_of_syn[k] - r->slot0];
+    x.w = w[k];
+    x.flags = (w[k] < 0.0f) ? 1u : 0u;
+  }
+  NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice,
+                        sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return NC_OK;
+}
+
+extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t* arrival,
+                                           int32_t* last_arrival, float* depol, uint64_t n) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
+  std::vector<nc_slot> hs;
+  std::vector<uint8_t> hst;
+  st = read_slots(sim, *r, &hs, &hst);
+  if (st != NC_OK) return st;
+  const int32_t s = (int32_t)r->steps_done;
+  const int32_t smod = s % 255;
+  for (uint64_t k = 0; k < n; ++k) {
+    const uint64_t j = r->slot_of_syn[k] - r->slot0;
+    const uint32_t code = hst[j];
+    if (arrival) {
+      if (code == 0) arrival[k] = 0;
+      else {
+        int32_t o = nc_stamp_offset(smod, code);
+        if (o > sim->T.r_rel) o -= 255;  // in flight: arrival in the future
+        arrival[k] = s - o;
+      }
+    }
+    if (last_arrival) last_arrival[k] = hs[j].last_arr;
+    if (depol) depol[k] = hs[j].depol;
+  }
+  return NC_OK;
+}
+
+extern "C" nc_status nc_read_raster(nc_sim* sim, int32_t net_id, uint32_t* steps, uint32_t* ids,
+                                    uint64_t cap, uint64_t* n_out) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!n_out) return fail(NC_ERR_ARG, "null argument");
+  if (!sim->raster_cap) return fail(NC_ERR_STATE, "raster logging is off (raster_capacity = 0)");
+  NC_HIP(hipSetDevice(sim->device));
+  uint32_t cnt = 0;
+  NC_HIP(hipMemcpyAsync(&cnt, sim->raster_cnt, 4, hipMemcpyDeviceToHost, sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  if (cnt > sim->raster_cap) return fail(NC_ERR_CAPACITY, "raster log overflowed; read it more often");
+  std::vector<uint32_t> buf(2ull * cnt);
+  if (cnt) NC_HIP(hipMemcpy(buf.data(), sim->raster, 8ull * cnt, hipMemcpyDeviceToHost));
+  // Keep other networks' records; hand back this network's, sorted.
+  std::vector<std::pair<uint32_t, uint32_t>> mine, rest;
+  const uint32_t p0 = r->q0 * 64, p1 = p0 + r->nq * 64;
+  for (uint32_t k = 0; k < cnt; ++k) {
+    uint32_t pos = buf[2 * k], s = buf[2 * k + 1];
+    if (pos >= p0 && pos < p1) mine.push_back({s, r->id_of_pos[pos - p0]});
+    else rest.push_back({pos, s});
+  }
+  std::sort(mine.begin(), mine.end());
+  if (mine.size() > cap) return fail(NC_ERR_CAPACITY, "output buffer too small for the raster");
+  for (size_t k = 0; k < mine.size(); ++k) {
+    if (steps) steps[k] = mine[k].first;
+    if (ids) ids[k] = mine[k].second;
+  }
+  *n_out = mine.size();
+  std::vector<uint32_t> back;
+  for (auto& x : rest) {
+    back.push_back(x.first);
+    back.push_back(x.second);
+  }
+  uint32_t nrest = (uint32_t)rest.size();
+  if (nrest) NC_HIP(hipMemcpy(sim->raster, back.data(), back.size() * 4, hipMemcpyHostToDevice));
+  NC_HIP(hipMemcpy(sim->raster_cnt, &nrest, 4, hipMemcpyHostToDevice));
+  return NC_OK;
+}
+
+extern "C" nc_status nc_reset_activities(nc_sim* sim, int32_t net_id) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  NC_HIP(hipSetDevice(sim->device));
+  NC_HIP(hipMemsetAsync(sim->firings + (size_t)r->q0 * 64, 0, (size_t)r->nq * 64 * 4, sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return NC_OK;
+}
+
+extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t out[8]) {
+  if (!sim || !out) return fail(NC_ERR_ARG, "null argument");
+  NC_HIP(hipSetDevice(sim->device));
+  std::vector<unsigned long long> h(NC_CTR_ROWS * NC_NCTR);
+  NC_HIP(hipMemcpyAsync(h.data(), sim->ctr, h.size() * 8, hipMemcpyDeviceToHost, sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  for (int k = 0; k < NC_NCTR; ++k) {
+    uint64_t v = 0;
+    for (int r = 0; r < NC_CTR_ROWS; ++r) v += h[r * NC_NCTR + k];
+    out[k] = v;
+  }
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_synchronize(nc_sim* sim) {
+  if (!sim) return fail(NC_ERR_ARG, "null sim");
+  NC_HIP(hipSetDevice(sim->device));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return NC_OK;
+}
+
+extern "C" nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask) {
+  if (!sim) return fail(NC_ERR_ARG, "null sim");
+  sim->kernel_timing = kernel_mask & 0xF;
+  for (int k = 0; k < 4; ++k) {
+    sim->k_launch[k] = 0;
+    sim->k_ms[k] = 0.0;
+  }
+  return NC_OK;
+}
+
+extern "C" nc_status nc_read_kernel_timing(nc_sim* sim, uint32_t kernel, uint64_t* launches,
+                                           double* total_ms) {
+  if (!sim) return fail(NC_ERR_ARG, "null sim");
+  if (kernel > 3) return fail(NC_ERR_RANGE, "kernel index out of range");
+  if (launches) *launches = sim->k_launch[kernel];
+  if (total_ms) *total_ms = sim->k_ms[kernel];
+  return NC_OK;
+}

@@ -1,0 +1,209 @@
+// ref_harness.cpp -- headless driver around the reference's simulation core.
+//
+// TEST INFRASTRUCTURE ONLY.  Built by oracle/Makefile together with the
+// reference's own, unmodified src/NeuCor.cpp (compiled where it lies under
+// the reference checkout) into oracle/_ref/ref_harness.  Used to pin the C
+// oracle bit-exactly on tie-free micro-networks, for statistical comparison,
+// and as the CPU baseline.  Never linked into the product.
+//
+// Access to protected/private members goes through the reference's own
+// friend declaration `friend class NeuCor_Renderer` (src/NeuCor.h:98, :270):
+// this file defines a class of that name with static accessors only.
+//
+// rand() is supplied here so that the reference's background-fire loop
+// (src/NeuCor.cpp:604-607) is scripted: it fires exactly the neurons the
+// scenario asks for, at offset runSpeed (i.e. on the next step of the grid),
+// and nothing else.  In --bench mode every draw also calls glibc's rand() so
+// that the loop keeps its real cost.
+//
+// Scenario file (little endian):
+//   u32 magic 'NCS1', u32 n, u64 S, u32 steps, u32 mode_flags,
+//   f32 dt, lr, pre_f, post_f
+//   i64 in_ptr[n+1]; u32 in_src[S]; f32 w[S]; u8 delay_steps[S]
+//   per step: u32 n_in, u32 ids[n_in]; u32 n_bg, u32 ids[n_bg]
+// Trace output (mode 0): u32 magic 'NCO1', n, steps, pad;
+//   per step: u32 n_fired, u32 ids[n_fired], f32 V[n]
+//   then f32 w[S], fire_t[S], last_arr[S] (synapse order of the file),
+//   f32 last_fire[n]
+#include <dlfcn.h>
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <chrono>
+#include <deque>
+#include <vector>
+
+#include "NeuCor.h"
+
+// ---- scripted rand() -------------------------------------------------------
+static std::deque<int> g_script;
+static bool g_bench = false;
+typedef int (*rand_fn)(void);
+static rand_fn g_real_rand = nullptr;
+
+extern "C" int rand(void) {
+  if (g_bench) {
+    if (!g_real_rand) g_real_rand = (rand_fn)dlsym(RTLD_NEXT, "rand");
+    (void)g_real_rand();
+  }
+  if (g_script.empty()) return 1;  // 1 % period != 0: no background fire
+  int v = g_script.front();
+  g_script.pop_front();
+  return v;
+}
+
+class NeuCor_Renderer {
+ public:
+  static std::deque<Neuron>& neurons(NeuCor& b) { return b.neurons; }
+  static Synapse* syn(NeuCor& b, size_t from, size_t to) { return b.getSynapse(from, to); }
+  static void set_length(Synapse& s, float L) { s.length = L; }
+  static float fire_t(const Synapse& s) { return s.AP_fireTime; }
+  static float last_arr(const Synapse& s) { return s.lastSpikeArrival; }
+};
+
+struct Scenario {
+  uint32_t n = 0, steps = 0, flags = 0;
+  uint64_t S = 0;
+  float dt = 1, lr = 1, pre_f = 0.13f, post_f = 0.30f;
+  std::vector<int64_t> in_ptr;
+  std::vector<uint32_t> in_src;
+  std::vector<float> w;
+  std::vector<uint8_t> delay;
+  std::vector<std::vector<uint32_t>> in_ev, bg_ev;
+};
+
+static bool rd(FILE* f, void* p, size_t n) { return fread(p, 1, n, f) == n; }
+
+static bool load(const char* path, Scenario* sc) {
+  FILE* f = fopen(path, "rb");
+  if (!f) return false;
+  uint32_t magic = 0;
+  bool ok = rd(f, &magic, 4) && magic == 0x3153434eu && rd(f, &sc->n, 4) && rd(f, &sc->S, 8) &&
+            rd(f, &sc->steps, 4) && rd(f, &sc->flags, 4) && rd(f, &sc->dt, 4) && rd(f, &sc->lr, 4) &&
+            rd(f, &sc->pre_f, 4) && rd(f, &sc->post_f, 4);
+  if (ok) {
+    sc->in_ptr.resize(sc->n + 1);
+    sc->in_src.resize(sc->S);
+    sc->w.resize(sc->S);
+    sc->delay.resize(sc->S);
+    ok = rd(f, sc->in_ptr.data(), 8 * (sc->n + 1)) && rd(f, sc->in_src.data(), 4 * sc->S) &&
+         rd(f, sc->w.data(), 4 * sc->S) && rd(f, sc->delay.data(), sc->S);
+  }
+  sc->in_ev.resize(sc->steps);
+  sc->bg_ev.resize(sc->steps);
+  for (uint32_t s = 0; ok && s < sc->steps; ++s) {
+    uint32_t k = 0;
+    ok = rd(f, &k, 4);
+    sc->in_ev[s].resize(k);
+    ok = ok && rd(f, sc->in_ev[s].data(), 4 * k);
+    ok = ok && rd(f, &k, 4);
+    sc->bg_ev[s].resize(k);
+    ok = ok && rd(f, sc->bg_ev[s].data(), 4 * k);
+  }
+  fclose(f);
+  return ok;
+}
+
+int main(int argc, char** argv) {
+  if (argc < 3) {
+    fprintf(stderr, "usage: ref_harness [--bench] scenario.bin out.bin\n");
+    return 2;
+  }
+  int a = 1;
+  if (strcmp(argv[a], "--bench") == 0) {
+    g_bench = true;
+    ++a;
+  }
+  if (argc - a < 2) return 2;
+  Scenario sc;
+  if (!load(argv[a], &sc)) {
+    fprintf(stderr, "cannot read scenario %s\n", argv[a]);
+    return 1;
+  }
+  auto t_build0 = std::chrono::steady_clock::now();
+  NeuCor brain(0);
+  brain.runAll = true;
+  brain.runSpeed = sc.dt;
+  brain.learningRate = sc.lr;
+  brain.presynapticFactor = sc.pre_f;
+  brain.postsynapticFactor = sc.post_f;
+  for (uint32_t i = 0; i < sc.n; ++i) brain.createNeuron(coord3{0.0f, 0.0f, (float)i});
+  std::deque<Neuron>& neu = NeuCor_Renderer::neurons(brain);
+  for (uint32_t tg = 0; tg < sc.n; ++tg) {
+    for (int64_t k = sc.in_ptr[tg]; k < sc.in_ptr[tg + 1]; ++k) {
+      uint32_t src = sc.in_src[k];
+      brain.createSynapse(tg, src, sc.w[k]);
+      // Synapse delay is length * AP_speed (src/NeuCor.cpp:733); AP_speed = 2.
+      NeuCor_Renderer::set_length(neu[src].outSynapses.back(), (float)sc.delay[k] * sc.dt / 2.0f);
+    }
+  }
+  // Repair the sign-class flag lost by vector growth (src/NeuCor.cpp:487-524).
+  for (auto& n : neu)
+    for (auto& s : n.outSynapses) s.setWeight(s.getWeight());
+  double build_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_build0).count();
+
+  FILE* out = fopen(argv[a + 1], "wb");
+  if (!out) return 1;
+  if (g_bench) {
+    uint64_t fires = 0;
+    std::vector<float> prev(sc.n);
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t s = 0; s < sc.steps; ++s) {
+      brain.run();
+      for (uint32_t id : sc.in_ev[s]) neu[id].fire();
+    }
+    double run_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    for (auto& n : neu)
+      if (n.lastFire == n.lastFire) ++fires;
+    fprintf(out, "{\"build_s\": %.6f, \"run_s\": %.6f, \"steps\": %u, \"neurons\": %u, \"synapses\": %llu, "
+                 "\"neurons_fired_ever\": %llu}\n",
+            build_s, run_s, sc.steps, sc.n, (unsigned long long)sc.S, (unsigned long long)fires);
+    fclose(out);
+    return 0;
+  }
+  uint32_t hdr[4] = {0x314f434eu, sc.n, sc.steps, 0u};
+  fwrite(hdr, 4, 4, out);
+  std::vector<float> V(sc.n);
+  std::vector<uint32_t> fired;
+  for (uint32_t s = 0; s < sc.steps; ++s) {
+    // Background fires of step s+1 are drawn by the run() that advances to
+    // it: hit (0), neuron (id), offset randomUnit() * runSpeed = runSpeed.
+    for (uint32_t id : sc.bg_ev[s]) {
+      g_script.push_back(0);
+      g_script.push_back((int)id);
+      g_script.push_back(RAND_MAX);
+    }
+    brain.run();
+    g_script.clear();
+    const float t = brain.getTime();
+    for (uint32_t id : sc.in_ev[s]) neu[id].fire();
+    for (auto& n : neu) n.run();  // what runAll does at the start of the next run()
+    fired.clear();
+    for (uint32_t i = 0; i < sc.n; ++i) {
+      if (neu[i].lastFire == t) fired.push_back(i);
+      V[i] = neu[i].potential();
+    }
+    uint32_t nf = (uint32_t)fired.size();
+    fwrite(&nf, 4, 1, out);
+    fwrite(fired.data(), 4, nf, out);
+    fwrite(V.data(), 4, sc.n, out);
+  }
+  std::vector<float> w(sc.S), ft(sc.S), la(sc.S), lf(sc.n);
+  for (uint32_t tg = 0; tg < sc.n; ++tg) {
+    for (int64_t k = sc.in_ptr[tg]; k < sc.in_ptr[tg + 1]; ++k) {
+      Synapse* sy = NeuCor_Renderer::syn(brain, sc.in_src[k], tg);
+      w[k] = sy->getWeight();
+      ft[k] = NeuCor_Renderer::fire_t(*sy);
+      la[k] = NeuCor_Renderer::last_arr(*sy);
+    }
+  }
+  for (uint32_t i = 0; i < sc.n; ++i) lf[i] = neu[i].lastFire;
+  fwrite(w.data(), 4, sc.S, out);
+  fwrite(ft.data(), 4, sc.S, out);
+  fwrite(la.data(), 4, sc.S, out);
+  fwrite(lf.data(), 4, sc.n, out);
+  fclose(out);
+  return 0;
+}
