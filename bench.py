@@ -35,12 +35,14 @@ def neuron_kernel_bytes(n_neurons, n_syn, c0, c1, steps):
        4 B per active slot-step (latched depolarisation read)
       13 B per processed arrival (w read+write 8, last-arrival write 4,
                                   latch clear 1)
-      20 B per fired neuron (lastFire write 4, firings read+write 8,
-                             fired-list entry 4, raster/compaction 4)"""
-    A = (c1["active_slot_steps"] - c0["active_slot_steps"]) / steps
-    R = (c1["arrivals"] - c0["arrivals"]) / steps
-    F = (c1["fired"] - c0["fired"]) / steps
-    return 16.0 * n_neurons + 1.0 * n_syn + 4.0 * A + 13.0 * R + 20.0 * F, {"A": A, "R": R, "F": F}
+      12 B per incoming synapse of a fired neuron (post-side STDP:
+                                  last-arrival read 4, w read+write 8)
+      12 B per fired neuron (lastFire write 4, firings read+write 8)
+       8 B per 64 neurons (fired bitmask of the slice)"""
+    d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree")}
+    A, R, F, Fin = d["active_slot_steps"], d["arrivals"], d["fired"], d["fired_in_degree"]
+    b = 16.0 * n_neurons + 1.0 * n_syn + 4.0 * A + 13.0 * R + 12.0 * Fin + 12.0 * F + 8.0 * n_neurons / 64
+    return b, {"A": A, "R": R, "F": F, "F_in": Fin}
 
 
 def drive_fires(drive, ids, groups, n_corr_groups, steps, keep):
@@ -127,6 +129,8 @@ def main():
     ap.add_argument("--workload", default="plain-1m")
     ap.add_argument("--chunk", type=int, default=500)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-kernel-timing", action="store_true",
+                    help="leave HIP events out of the timed region (for external profilers)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: every box in this setup has a single MI355X; "
@@ -154,7 +158,8 @@ def main():
         done += k
     sim.synchronize()
     c0 = sim.counters()
-    sim.set_kernel_timing(1 << 1)  # HIP events around every neuron-step launch
+    if not args.no_kernel_timing:
+        sim.set_kernel_timing(1 << 1)  # HIP events around every neuron-step launch
     t_start = time.perf_counter()
     done = 0
     while done < args.steps:
@@ -170,8 +175,8 @@ def main():
 
     value = S * args.steps / elapsed
     bytes_launch, ev = neuron_kernel_bytes(N, S, c0, c1, args.steps)
-    avg_launch_s = (k_ms / 1000.0) / max(launches, 1)
-    achieved = bytes_launch / avg_launch_s / 1e9
+    avg_launch_s = (k_ms / 1000.0) / launches if launches else None
+    achieved = bytes_launch / avg_launch_s / 1e9 if avg_launch_s else None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
@@ -201,9 +206,9 @@ def main():
                    "stdp": True, "seed": 0, "fired_per_step": ev["F"], "rate_per_step": rate,
                    "arrivals_per_step": ev["R"], "active_slots_per_step": ev["A"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": "k_neuron", "algorithmic_bytes_per_launch": bytes_launch,
-                     "avg_launch_us": avg_launch_s * 1e6, "launches": launches},
+                     "avg_launch_us": avg_launch_s * 1e6 if avg_launch_s else None, "launches": launches},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -39,10 +39,11 @@
 #define NC_MAX_STEPS (1 << 29)
 
 // ---- stamp codes -----------------------------------------------------------
-// A synapse carries at most one spike (src/NeuCor.cpp:728).  Its in-flight
-// state is one byte: 0 = idle, else (arrival_step mod 255) + 1.  The offset
-// o = s - arrival is recovered modulo 255; it is unambiguous while
-// max_delay + release_offset < 255 (checked on the host).
+// A synapse carries at most one spike (src/NeuCor.cpp:728).  Once the spike
+// has arrived, and until its latch releases, its state is one byte: 0 = idle,
+// else (arrival_step mod 255) + 1; the offset o = s - arrival is recovered
+// modulo 255, unambiguous while the release window is shorter than 255
+// steps.  Spikes still in flight wait in their target slice's pending list.
 NC_HD uint32_t nc_stamp_code(int32_t arrival_step) {
   return (uint32_t)(arrival_step % 255) + 1u;
 }

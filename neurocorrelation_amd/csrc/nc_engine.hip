@@ -230,27 +230,38 @@ struct nc_sim {
   nc_slot* slots = nullptr;
   float* V = nullptr;
   int32_t* last_fire = nullptr;
-  uint32_t *firings = nullptr, *degw = nullptr, *nid = nullptr, *in_group = nullptr;
+  uint32_t *firings = nullptr, *nid = nullptr;
+  uint2* ndw = nullptr;
   uint32_t *out_beg = nullptr, *out_cnt = nullptr, *out_slot = nullptr;
   uint8_t* out_delay = nullptr;
-  uint32_t *slice_off = nullptr, *slice_net = nullptr, *active = nullptr;
+  int32_t* out_arr = nullptr;
+  uint32_t* out_tq = nullptr;
+  uint2* pend = nullptr;
+  uint32_t* pend_cnt = nullptr;
+  uint32_t *slice_off = nullptr, *slice_net = nullptr;
+  nc_slice_rec* active = nullptr;
   uint32_t *forced_in = nullptr, *forced_bg = nullptr;
-  uint32_t *fired = nullptr, *fired_cnt = nullptr;
+  unsigned long long* fmask = nullptr;
+
+  uint64_t ctr_rows = 0;
   uint32_t *raster = nullptr, *raster_cnt = nullptr;
   unsigned long long* ctr = nullptr;
   nc_netdesc* d_nets = nullptr;
   float *t_pre = nullptr, *t_post = nullptr, *w_ap = nullptr;
   DevBuf<nc_group> d_groups;
+  DevBuf<uint32_t> d_gfire;
   DevBuf<uint32_t> d_thr;
   DevBuf<uint32_t> d_ev_pos;
   DevBuf<uint8_t> d_ev_kind;
 
   std::vector<nc_group> groups;
   std::vector<uint32_t> thr;
-  std::vector<uint32_t> active_host;
+  std::vector<nc_slice_rec> active_host;
+  std::vector<uint32_t> slice_off_host;
   bool active_dirty = true, groups_dirty = true;
   std::vector<uint32_t> slice_net_host;
   uint64_t raster_read = 0;
+  uint64_t host_neuron_steps = 0, host_synapse_steps = 0;
   // Host staging of the per-call uploads; reused only after upload_done.
   std::vector<nc_netdesc> h_nds;
   std::vector<uint32_t> h_ev_pos;
@@ -290,9 +301,8 @@ nc_step_args base_args(nc_sim* sim) {
   a.V = sim->V;
   a.last_fire = sim->last_fire;
   a.firings = sim->firings;
-  a.degw = sim->degw;
+  a.ndw = sim->ndw;
   a.nid = sim->nid;
-  a.in_group = sim->in_group;
   a.slice_off = sim->slice_off;
   a.slice_net = sim->slice_net;
   a.active_slices = sim->active;
@@ -311,6 +321,9 @@ nc_step_args base_args(nc_sim* sim) {
   a.w_ap = sim->w_ap;
   a.len_pre = (int32_t)sim->T.t_pre.size();
   a.len_post = (int32_t)sim->T.t_post.size();
+  a.tabs_in_lds = (a.len_pre <= NC_LDS_TAB && a.len_post <= NC_LDS_TAB) ? 1 : 0;
+  for (int k = 0; k < NC_WAP_K; ++k)
+    a.w_ap_k[k] = k < (int)sim->T.w_ap.size() ? sim->T.w_ap[k] : 0.0f;
   a.dt = sim->T.dt;
   a.P = sim->T.P;
   a.depol_k = sim->T.depol_k;
@@ -322,6 +335,11 @@ nc_step_args base_args(nc_sim* sim) {
   a.out_cnt = sim->out_cnt;
   a.out_slot = sim->out_slot;
   a.out_delay = sim->out_delay;
+  a.out_arr = sim->out_arr;
+  a.out_tq = sim->out_tq;
+  a.pend = sim->pend;
+  a.pend_cnt = sim->pend_cnt;
+  a.fmask = sim->fmask;
   return a;
 }
 
@@ -401,35 +419,41 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMalloc(&sim->V, P * 4), "hipMalloc V");
     H(hipMalloc(&sim->last_fire, P * 4), "hipMalloc lastFire");
     H(hipMalloc(&sim->firings, P * 4), "hipMalloc firings");
-    H(hipMalloc(&sim->degw, P * 4), "hipMalloc degw");
+    H(hipMalloc(&sim->ndw, P * sizeof(uint2)), "hipMalloc ndw");
     H(hipMalloc(&sim->nid, P * 4), "hipMalloc nid");
-    H(hipMalloc(&sim->in_group, P * 4), "hipMalloc in_group");
     H(hipMalloc(&sim->out_beg, P * 4), "hipMalloc out_beg");
     H(hipMalloc(&sim->out_cnt, P * 4), "hipMalloc out_cnt");
     H(hipMalloc(&sim->out_slot, sim->max_out * 4), "hipMalloc out_slot");
     H(hipMalloc(&sim->out_delay, sim->max_out), "hipMalloc out_delay");
+    H(hipMalloc(&sim->out_arr, sim->max_out * 4), "hipMalloc out_arr");
+    H(hipMalloc(&sim->out_tq, sim->max_out * 4), "hipMalloc out_tq");
+    H(hipMalloc(&sim->pend, sim->max_slots * sizeof(uint2)), "hipMalloc pend");
+    H(hipMalloc(&sim->pend_cnt, Q * 4), "hipMalloc pend_cnt");
     H(hipMalloc(&sim->slice_off, Q * 4), "hipMalloc slice_off");
     H(hipMalloc(&sim->slice_net, Q * 4), "hipMalloc slice_net");
-    H(hipMalloc(&sim->active, Q * 4), "hipMalloc active");
+    H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
     H(hipMalloc(&sim->forced_in, (P / 32) * 4), "hipMalloc forced_in");
     H(hipMalloc(&sim->forced_bg, (P / 32) * 4), "hipMalloc forced_bg");
-    H(hipMalloc(&sim->fired, 2 * P * 4), "hipMalloc fired");
-    H(hipMalloc(&sim->fired_cnt, 64), "hipMalloc fired_cnt");
+    H(hipMalloc(&sim->fmask, Q * 8), "hipMalloc fmask");
+
     H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
     if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
-    H(hipMalloc(&sim->ctr, NC_CTR_ROWS * NC_NCTR * 8), "hipMalloc ctr");
+    sim->ctr_rows = Q / 4 + 2;
+    H(hipMalloc(&sim->ctr, sim->ctr_rows * NC_NCTR * 8), "hipMalloc ctr");
     H(hipMalloc(&sim->d_nets, sim->max_nets * sizeof(nc_netdesc)), "hipMalloc nets");
   }
   if (st == NC_OK) {
     H(hipMemsetAsync(sim->stamp, 0, sim->max_slots, sim->stream), "memset");
-    H(hipMemsetAsync(sim->degw, 0, P * 4, sim->stream), "memset");
+    H(hipMemsetAsync(sim->ndw, 0, P * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
-    H(hipMemsetAsync(sim->fired_cnt, 0, 64, sim->stream), "memset");
+    H(hipMemsetAsync(sim->fmask, 0, Q * 8, sim->stream), "memset");
+
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
-    H(hipMemsetAsync(sim->ctr, 0, NC_CTR_ROWS * NC_NCTR * 8, sim->stream), "memset");
+    H(hipMemsetAsync(sim->ctr, 0, sim->ctr_rows * NC_NCTR * 8, sim->stream), "memset");
     std::vector<uint32_t> none(Q, NC_SLICE_NONE);
     sim->slice_net_host = none;
+    sim->slice_off_host.assign(Q, 0u);
     H(hipMemcpyAsync(sim->slice_net, none.data(), Q * 4, hipMemcpyHostToDevice, sim->stream), "memcpy");
     H(hipStreamSynchronize(sim->stream), "sync");
   }
@@ -448,12 +472,12 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (!sim) return;
   (void)hipSetDevice(sim->device);
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
-  void* ptrs[] = {sim->stamp, sim->slots, sim->V, sim->last_fire, sim->firings, sim->degw,
-                  sim->nid, sim->in_group, sim->out_beg, sim->out_cnt, sim->out_slot,
-                  sim->out_delay, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
-                  sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
+  void* ptrs[] = {sim->stamp, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
+                  sim->nid, sim->out_beg, sim->out_cnt, sim->out_slot,
+                  sim->out_delay, sim->out_arr, sim->out_tq, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->forced_bg, sim->fmask, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
-                  sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p};
+                  sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : sim->ev_pool) (void)hipEventDestroy(e);
@@ -500,8 +524,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
              p.post_trace_decay != sim->T.post_decay) {
     return fail(NC_ERR_ARG, "all networks of one nc_sim share dt and the trace decays");
   }
-  if ((uint32_t)net->max_delay + (uint32_t)sim->T.r_rel >= 254u)
-    return fail(NC_ERR_ARG, "max delay too long for this dt (delay + release window must be < 254 steps)");
+  if ((uint32_t)sim->T.r_rel >= 250u) return fail(NC_ERR_ARG, "dt too fine: the release window must be < 250 steps");
   if (net->max_in_degree > NC_DEG_MASK) return fail(NC_ERR_ARG, "in-degree too large");
 
   // Free network id.
@@ -583,7 +606,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       o += outcnt[pl];
     }
   }
-  std::vector<uint32_t> oslot(S);
+  std::vector<uint32_t> oslot(S), otq(S);
   std::vector<uint8_t> odly(S);
   {
     std::vector<uint32_t> fillp(outbeg);
@@ -593,6 +616,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
         uint32_t e = fillp[sp]++ - (uint32_t)o0;
         oslot[e] = r.slot_of_syn[k];
         odly[e] = net->delay[k];
+        otq[e] = (uint32_t)q0 + L.pos_of_id[tg] / 64;
       }
     }
   }
@@ -630,6 +654,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       G.tbl_off = (uint32_t)sim->thr.size();
       G.period = T;
       G.local = g;
+      G.pad = (uint32_t)id;
       const double amp = g < ncorr ? drive->amp_corr : drive->amp_ind;
       for (uint32_t k = 0; k < T; ++k) {
         double pr = (double)drive->p0 * (1.0 + amp * sin(2.0 * M_PI * (double)k / (double)T));
@@ -668,20 +693,29 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   NC_HIP(hipMemcpyAsync(sim->V + pos0, V0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->firings + pos0, zero.data(), npos * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->degw + pos0, degw.data(), npos * 4, hipMemcpyHostToDevice, st));
+  std::vector<uint2> ndwv(npos);
+  for (uint32_t pl = 0; pl < npos; ++pl) ndwv[pl] = make_uint2(degw[pl], ingrp[pl]);
+  NC_HIP(hipMemcpyAsync(sim->ndw + pos0, ndwv.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->nid + pos0, nidv.data(), npos * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->in_group + pos0, ingrp.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->out_beg + pos0, outbeg.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->out_cnt + pos0, outcnt.data(), npos * 4, hipMemcpyHostToDevice, st));
   if (S) {
     NC_HIP(hipMemcpyAsync(sim->out_slot + o0, oslot.data(), S * 4, hipMemcpyHostToDevice, st));
     NC_HIP(hipMemcpyAsync(sim->out_delay + o0, odly.data(), S, hipMemcpyHostToDevice, st));
+    std::vector<int32_t> never(S, NC_NEVER);
+    NC_HIP(hipMemcpyAsync(sim->out_arr + o0, never.data(), S * 4, hipMemcpyHostToDevice, st));
+    NC_HIP(hipStreamSynchronize(st));
+    NC_HIP(hipMemcpyAsync(sim->out_tq + o0, otq.data(), S * 4, hipMemcpyHostToDevice, st));
   }
+  NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, L.nq * 4, st));
   NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), L.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), L.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
   NC_HIP(hipStreamSynchronize(st));
-  for (uint32_t q = 0; q < L.nq; ++q) sim->slice_net_host[q0 + q] = (uint32_t)id;
+  for (uint32_t q = 0; q < L.nq; ++q) {
+    sim->slice_net_host[q0 + q] = (uint32_t)id;
+    sim->slice_off_host[q0 + q] = slice_off[q];
+  }
   r.live = true;
   sim->active_dirty = true;
   *net_id = id;
@@ -776,10 +810,11 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     for (size_t i = 0; i < sim->nets.size(); ++i) {
       const NetRec& r = sim->nets[i];
       if (!r.live) continue;
-      for (uint32_t q = 0; q < r.nq; ++q) sim->active_host.push_back(r.q0 + q);
+      for (uint32_t q = r.q0; q < r.q0 + r.nq; ++q)
+        sim->active_host.push_back(nc_slice_rec{q, sim->slice_off_host[q], (uint32_t)i, 0u});
     }
     if (!sim->active_host.empty())
-      NC_HIP(hipMemcpyAsync(sim->active, sim->active_host.data(), sim->active_host.size() * 4,
+      NC_HIP(hipMemcpyAsync(sim->active, sim->active_host.data(), sim->active_host.size() * sizeof(nc_slice_rec),
                             hipMemcpyHostToDevice, st));
     sim->active_dirty = false;
   }
@@ -827,16 +862,21 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
   nc_step_args a = base_args(sim);
   a.groups = sim->d_groups.p;
   a.thr = sim->d_thr.p;
+  a.n_groups = (uint32_t)sim->groups.size();
+  a.gwords = (a.n_groups + 31) / 32;
+  if (a.n_groups) {
+    nc_status s1 = grow(&sim->d_gfire, (size_t)max_run * a.gwords, st);
+    if (s1 != NC_OK) return s1;
+    a.gfire = sim->d_gfire.p;
+    const uint64_t nthreads = (uint64_t)max_run * a.gwords * 32;
+    hipLaunchKernelGGL(k_groups, dim3((uint32_t)((nthreads + 255) / 256)), dim3(256), 0, st, a, max_run);
+    NC_HIP(hipGetLastError());
+  }
   const uint32_t blocks1 = (a.n_active_slices + 3) / 4;
-  const uint32_t blocks_w = 1024;
   size_t nev_used = 0;
   std::vector<int> timed_kinds;
   for (uint32_t i = 0; i < max_run; ++i) {
-    const uint32_t b = (uint32_t)(sim->gstep & 1);
     a.chunk_i = (int32_t)i;
-    a.fired = sim->fired + (uint64_t)b * sim->max_pos;
-    a.fired_cnt = sim->fired_cnt + b;
-    a.fired_cnt_next = sim->fired_cnt + (b ^ 1);
     a.n_ev = ev_ptr[i + 1] - ev_ptr[i];
     a.ev_pos = sim->d_ev_pos.p + ev_ptr[i];
     a.ev_kind = sim->d_ev_kind.p + ev_ptr[i];
@@ -863,8 +903,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     nc_status s1 = NC_OK;
     if (a.n_ev) s1 = launch(0, k_drive, (a.n_ev + 255) / 256);
     if (s1 == NC_OK) s1 = launch(1, k_neuron, blocks1);
-    if (s1 == NC_OK) s1 = launch(2, k_post, blocks_w);
-    if (s1 == NC_OK) s1 = launch(3, k_fanout, blocks_w);
+    if (s1 == NC_OK) s1 = launch(3, k_fanout, blocks1);
     if (s1 != NC_OK) return s1;
     sim->gstep++;
   }
@@ -880,6 +919,8 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     NetRec& r = sim->nets[i];
     if (!r.live) continue;
     uint32_t run = (uint32_t)nds[i].remaining;
+    sim->host_neuron_steps += (uint64_t)run * r.n;
+    sim->host_synapse_steps += (uint64_t)run * r.S;
     r.steps_done += run;
     if (r.steps_left != UINT32_MAX) r.steps_left -= run;
   }
@@ -1034,18 +1075,27 @@ extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t*
   std::vector<uint8_t> hst;
   st = read_slots(sim, *r, &hs, &hst);
   if (st != NC_OK) return st;
+  // spikes still in flight: the pending lists of the network's slices
+  std::vector<int32_t> inflight(r->nslots, 0);
+  {
+    std::vector<uint32_t> cnt(r->nq);
+    std::vector<uint2> ent(r->nslots);
+    NC_HIP(hipMemcpy(cnt.data(), sim->pend_cnt + r->q0, r->nq * 4, hipMemcpyDeviceToHost));
+    NC_HIP(hipMemcpy(ent.data(), sim->pend + r->slot0, r->nslots * sizeof(uint2), hipMemcpyDeviceToHost));
+    for (uint32_t q = 0; q < r->nq; ++q) {
+      const uint64_t b = sim->slice_off_host[r->q0 + q] - r->slot0;
+      for (uint32_t t = 0; t < cnt[q]; ++t) inflight[ent[b + t].x - r->slot0] = (int32_t)ent[b + t].y;
+    }
+  }
   const int32_t s = (int32_t)r->steps_done;
   const int32_t smod = s % 255;
   for (uint64_t k = 0; k < n; ++k) {
     const uint64_t j = r->slot_of_syn[k] - r->slot0;
     const uint32_t code = hst[j];
     if (arrival) {
-      if (code == 0) arrival[k] = 0;
-      else {
-        int32_t o = nc_stamp_offset(smod, code);
-        if (o > sim->T.r_rel) o -= 255;  // in flight: arrival in the future
-        arrival[k] = s - o;
-      }
+      if (inflight[j]) arrival[k] = inflight[j];
+      else if (code == 0) arrival[k] = 0;
+      else arrival[k] = s - nc_stamp_offset(smod, code);
     }
     if (last_arrival) last_arrival[k] = hs[j].last_arr;
     if (depol) depol[k] = hs[j].depol;
@@ -1106,14 +1156,16 @@ extern "C" nc_status nc_reset_activities(nc_sim* sim, int32_t net_id) {
 extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t out[8]) {
   if (!sim || !out) return fail(NC_ERR_ARG, "null argument");
   NC_HIP(hipSetDevice(sim->device));
-  std::vector<unsigned long long> h(NC_CTR_ROWS * NC_NCTR);
+  std::vector<unsigned long long> h(sim->ctr_rows * NC_NCTR);
   NC_HIP(hipMemcpyAsync(h.data(), sim->ctr, h.size() * 8, hipMemcpyDeviceToHost, sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
   for (int k = 0; k < NC_NCTR; ++k) {
     uint64_t v = 0;
-    for (int r = 0; r < NC_CTR_ROWS; ++r) v += h[r * NC_NCTR + k];
+    for (uint64_t r = 0; r < sim->ctr_rows; ++r) v += h[r * NC_NCTR + k];
     out[k] = v;
   }
+  out[0] = sim->host_neuron_steps;
+  out[1] = sim->host_synapse_steps;
   return NC_OK;
 }
 

@@ -2,14 +2,21 @@
 //
 // One step s of every active network is, in order:
 //   k_drive   host-scheduled forced fires -> bitmasks       (only when present)
-//   k_neuron  one lane per neuron: charge_insynapses, charge_passive,
-//             threshold / forced fire, deferred arrival STDP, AP waveform,
-//             latch release; ballot-compacts the fired neurons
-//   k_post    one wave per fired neuron: post-side STDP over its incoming
-//             synapses (the inSynapses loop of Neuron::fire)
-//   k_fanout  one wave per fired neuron: Synapse::fire on its outgoing
-//             synapses through the out->in slot map, latch permitting
-// Kernel boundaries order the phases; no atomics touch weights.
+//   k_neuron  one lane per neuron (one wave per 64-neuron slice): moves the
+//             slice's spikes arriving at s from its pending list into the
+//             stamp bytes; charge_insynapses, charge_passive, threshold /
+//             forced fire, deferred arrival STDP, AP waveform, latch
+//             release; then the wave applies post-side STDP to the incoming
+//             synapses of each of its fired lanes and writes the slice's
+//             fired bitmask
+//   k_fanout  waves over the slices with fired lanes: Synapse::fire on
+//             their outgoing synapses (source-order latch copy, out->in
+//             slot map), appending accepted spikes to the target slice's
+//             pending list
+// The kernel boundary orders the phases.  No atomics in the step: every
+// synapse slot has one writer per phase (its target's lane, then its
+// source's wave), fired lists are per-slice ballots, counters are owned by
+// the block that adds to them.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -19,8 +26,9 @@
 #define NC_DEG_INPUT 0x01000000u
 #define NC_DEG_VALID 0x02000000u
 #define NC_SLICE_NONE 0xFFFFFFFFu
+#define NC_LDS_TAB 2048  // trace-table entries staged in LDS per block
+#define NC_WAP_K 8       // AP-table entries passed as kernel arguments
 #define NC_NCTR 8
-#define NC_CTR_ROWS 64
 
 // Per-network descriptor, rewritten by the host before each batch of steps.
 struct nc_netdesc {
@@ -33,11 +41,19 @@ struct nc_netdesc {
   uint32_t pad[6];
 };
 
+// One record per active slice (one wave): its index, first slot, network.
+struct nc_slice_rec {
+  uint32_t q;
+  uint32_t slot_off;
+  uint32_t net;
+  uint32_t pad;
+};
+
 struct nc_group {
   uint32_t tbl_off;  // first threshold entry
   uint32_t period;   // entries
   uint32_t local;    // group index inside its network (Philox counter word)
-  uint32_t pad;
+  uint32_t pad;      // network id
 };
 
 struct nc_step_args {
@@ -46,30 +62,32 @@ struct nc_step_args {
   float* V;
   int32_t* last_fire;
   uint32_t* firings;
-  const uint32_t* degw;
+  const uint2* ndw;  // per position: x = in-degree | flags, y = input group
   const uint32_t* nid;
-  const uint32_t* in_group;
   const uint32_t* slice_off;
   const uint32_t* slice_net;
-  const uint32_t* active_slices;
+  const nc_slice_rec* active_slices;
   uint32_t n_active_slices;
   const nc_netdesc* nets;
   const nc_group* groups;
   const uint32_t* thr;
+  uint32_t* gfire;       // [chunk step][group words]: group spike trains of this batch
+  uint32_t gwords;       // words per step in gfire
+  uint32_t n_groups;
   uint32_t* forced_in;
   uint32_t* forced_bg;
   int32_t use_forced;
-  uint32_t* fired;      // this step's list
-  uint32_t* fired_cnt;  // this step's count
-  uint32_t* fired_cnt_next;
+  unsigned long long* fmask;  // fired lanes of each active slice (by active-list index), this step
   uint32_t* raster;  // (pos, step) pairs
   uint32_t* raster_cnt;
   uint32_t raster_cap;
-  unsigned long long* ctr;  // [NC_CTR_ROWS][NC_NCTR]
+  unsigned long long* ctr;  // [grid blocks][NC_NCTR], owned per block
   const float* t_pre;
   const float* t_post;
   const float* w_ap;
   int32_t len_pre, len_post;
+  int32_t tabs_in_lds;  // both trace tables fit NC_LDS_TAB entries
+  float w_ap_k[NC_WAP_K];  // the AP table when it has <= NC_WAP_K entries
   float dt, P, depol_k;
   double c0, e;
   int32_t d_ref, r_rel;
@@ -78,6 +96,11 @@ struct nc_step_args {
   const uint32_t* out_cnt;
   const uint32_t* out_slot;
   const uint8_t* out_delay;
+  int32_t* out_arr;   // source order: arrival step of the synapse's last accepted spike
+                      // (NC_NEVER: none); its latch holds until arrival + release window
+  const uint32_t* out_tq;  // target slice of each out-synapse
+  uint2* pend;             // per slice, at its slot range: (slot, arrival step) of spikes in flight
+  uint32_t* pend_cnt;      // per slice: entries in its pending list
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
   uint32_t n_ev;
@@ -127,9 +150,35 @@ __device__ __forceinline__ uint32_t nc_wave_append(bool pred, uint32_t* cnt) {
   return pred ? base + (uint32_t)__builtin_popcountll(below) : ~0u;
 }
 
-__device__ __forceinline__ void nc_count(unsigned long long* ctr, int k, unsigned long long v) {
-  v = nc_wave_sum(v);
-  if (nc_lane() == 0 && v) atomicAdd(&ctr[(blockIdx.x & (NC_CTR_ROWS - 1)) * NC_NCTR + k], v);
+// Adds per-lane counts into the block's own counter row: a wave sum, then
+// one fire-and-forget atomic per counter from lane 0 (rows are per block,
+// so they see at most 4 writers).
+__device__ __forceinline__ void nc_count32(unsigned long long* ctr, int k, uint32_t v) {
+  for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
+  if (nc_lane() == 0 && v) atomicAdd(&ctr[(size_t)blockIdx.x * NC_NCTR + k], (unsigned long long)v);
+}
+
+// Group spike trains of a batch of steps: one Philox draw per (step, group),
+// shared by every member of the group.
+__global__ void k_groups(nc_step_args a, uint32_t n_steps) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_steps * a.gwords * 32u) return;
+  const uint32_t i = t / (a.gwords * 32u), g = t % (a.gwords * 32u);
+  bool fire = false;
+  if (g < a.n_groups) {
+    const nc_group G = a.groups[g];
+    const nc_netdesc* nd = &a.nets[G.pad];
+    if (i < (uint32_t)nd->remaining) {
+      const uint32_t s = (uint32_t)(nd->base_step + (int32_t)i + 1);
+      const uint64_t seed = ((uint64_t)nd->drive_hi << 32) | nd->drive_lo;
+      const uint32_t u = nc_draw(seed, s, G.local, NC_STREAM_DRIVE).v[0];
+      fire = u < a.thr[G.tbl_off + s % G.period];
+    }
+  }
+  const unsigned long long m = __ballot(fire);
+  const uint32_t lane = nc_lane();
+  if (lane == 0) a.gfire[(size_t)i * a.gwords + (g >> 5)] = (uint32_t)m;
+  if (lane == 32) a.gfire[(size_t)i * a.gwords + (g >> 5)] = (uint32_t)(m >> 32);
 }
 
 // Host-scheduled forced fires of this step -> bitmasks read by k_neuron.
@@ -141,64 +190,164 @@ __global__ void k_drive(nc_step_args a) {
   atomicOr(&m[pos >> 5], 1u << (pos & 31));
 }
 
+#define NC_BATCH 8  // stamp words (4 slots each) loaded per lane before use: 32 slots
+
 // Neuron::run (src/NeuCor.cpp:619-641) for every neuron of the active slices.
-__global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (wave >= a.n_active_slices) return;
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_neuron(nc_step_args a) {
+  __shared__ float sh_pre[NC_LDS_TAB], sh_post[NC_LDS_TAB];
+  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t q = a.active_slices[wave];
-  const uint32_t net = a.slice_net[q];
-  const nc_netdesc* nd = &a.nets[net];
-  const int32_t remaining = nd->remaining;
-  if (a.chunk_i >= remaining) return;
-  const int32_t s = nd->base_step + a.chunk_i + 1;
+  const bool wave_in = wave < a.n_active_slices;
+  const nc_slice_rec rec = a.active_slices[wave_in ? wave : 0];
+  const nc_netdesc nd = a.nets[rec.net];
+  const uint32_t q = rec.q;
   const uint32_t pos = q * 64 + lane;
-  const uint32_t dw = a.degw[pos];
+  const bool live = wave_in && a.chunk_i < nd.remaining;
+  const uint2 nw = live ? a.ndw[pos] : make_uint2(0u, 0u);
+  float V = live ? a.V[pos] : 0.0f;
+  int32_t lf = live ? a.last_fire[pos] : NC_NEVER;
+  const uint32_t pend_n = live ? a.pend_cnt[q] : 0u;
+  const int32_t s = nd.base_step + a.chunk_i + 1;
+  const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
-  const float lr = nd->lr, pre_f = nd->pre_f, post_f = nd->post_f;
+  const float lr = nd.lr, pre_f = nd.pre_f, post_f = nd.post_f;
+  if (a.tabs_in_lds) {
+    for (int32_t i = threadIdx.x; i < a.len_pre; i += blockDim.x) sh_pre[i] = a.t_pre[i];
+    for (int32_t i = threadIdx.x; i < a.len_post; i += blockDim.x) sh_post[i] = a.t_post[i];
+  }
+  __syncthreads();
+  const float* tpre = a.tabs_in_lds ? sh_pre : a.t_pre;
+  const float* tpost = a.tabs_in_lds ? sh_post : a.t_post;
 
-  float V = a.V[pos];
-  int32_t lf = a.last_fire[pos];
   const int32_t smod = s % 255;
-  const uint32_t sbase = a.slice_off[q] + lane * 4;
+  const uint32_t sbase = rec.slot_off + lane * 4;
   const uint32_t nch = (deg + 3) >> 2;
   uint32_t n_active = 0, n_arr = 0;
 
-  // charge_insynapses (src/NeuCor.cpp:688-700): ascending source order.
-  for (uint32_t c0 = 0; c0 < nch; c0 += 4) {
-    uint32_t wv[4];
+  // Spikes arriving now leave the pending list for the stamp bytes; the
+  // list is compacted in place (write index <= read index).
+  if (pend_n) {
+    uint2* pl = a.pend + rec.slot_off;
+    uint32_t kept = 0;
+    for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      uint2 e = make_uint2(0u, 0u);
+      if (t < pend_n) e = pl[t];
+      const bool arrive = t < pend_n && (int32_t)e.y == s;
+      if (arrive) a.stamp[e.x] = (uint8_t)nc_stamp_code(s);
+      const bool keep = t < pend_n && !arrive;
+      const unsigned long long km = __ballot(keep);
+      if (keep) pl[kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull))] = e;
+      kept += (uint32_t)__builtin_popcountll(km);
+    }
+    if (lane == 0) a.pend_cnt[q] = kept;
+  }
+
+  // charge_insynapses (src/NeuCor.cpp:688-700) in ascending source order,
+  // in groups of 4 x NC_BATCH stamp words (128 slots): the group's words are
+  // loaded NC_BATCH at a time and reduced to masks; then the active slots of
+  // the whole group are handled.
+  for (uint32_t c0 = 0; c0 < nch; c0 += 4 * NC_BATCH) {
+    // bit i of act/first/rel[h]: slot 64*h + i of the group is contributing /
+    // on its first step after arrival / on its last step before release
+    unsigned long long act[2] = {0ull, 0ull}, first[2] = {0ull, 0ull}, rel[2] = {0ull, 0ull};
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      wv[u] = (c0 + u < nch) ? *(const uint32_t*)(a.stamp + sbase + (c0 + u) * 256) : 0u;
+    for (int bt = 0; bt < 4; ++bt) {
+      uint32_t wv[NC_BATCH];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      uint32_t w4 = wv[u];
-      if (w4 == 0u) continue;
-      const uint32_t j0 = sbase + (c0 + u) * 256;
+      for (int u = 0; u < NC_BATCH; ++u) {
+        const uint32_t c = c0 + bt * NC_BATCH + u;
+        wv[u] = (c < nch) ? *(const uint32_t*)(a.stamp + sbase + c * 256) : 0u;
+      }
+      uint32_t am = 0, fm = 0, rm = 0;
 #pragma unroll
-      for (int b = 0; b < 4; ++b) {
-        const uint32_t code = (w4 >> (8 * b)) & 0xffu;
-        if (code == 0u) continue;
-        const int32_t o = nc_stamp_offset(smod, code);
-        if (o == 0 || o > a.r_rel) continue;  // arriving now, or still in flight
-        const uint32_t j = j0 + b;
-        nc_slot sl = a.slots[j];
-        const float inc = a.dt * sl.depol;
-        V = (float)((double)V + ((double)inc * a.c0) * a.e);
-        ++n_active;
-        if (o == 1 && sl.last_arr != s - 1) {
-          // The arrival of step s-1 (Synapse::run, src/NeuCor.cpp:718-726):
-          // lastSpikeArrival = s-1, then STDP with the synapse trace at 1 -> 0
-          // and the target trace of lastFire as it stood after step s-1.
-          const int32_t ar = s - 1;
-          const float tr_t = nc_lookup(a.t_post, a.len_post, ar - lf);
-          const float wn = nc_stdp(sl.w, sl.flags, 0.0f, tr_t, pre_f, post_f, lr);
-          a.slots[j].w = wn;
-          a.slots[j].last_arr = ar;
+      for (int u = 0; u < NC_BATCH; ++u) {
+        const uint32_t w4 = wv[u];
+        if (w4 == 0u) continue;
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const uint32_t code = (w4 >> (8 * b)) & 0xffu;
+          const int32_t o = nc_stamp_offset(smod, code);
+          const uint32_t bit = (uint32_t)(code != 0u && o >= 1 && o <= a.r_rel) << (4 * u + b);
+          am |= bit;
+          fm |= (o == 1) ? bit : 0u;
+          rm |= (o >= a.r_rel) ? bit : 0u;
+        }
+      }
+      act[bt >> 1] |= (unsigned long long)am << (32 * (bt & 1));
+      first[bt >> 1] |= (unsigned long long)fm << (32 * (bt & 1));
+      rel[bt >> 1] |= (unsigned long long)rm << (32 * (bt & 1));
+    }
+    // Contributions: the latched depolarisation of up to four active slots is
+    // fetched at once, then added in slot order.
+    while (act[0] | act[1]) {
+      uint32_t jj[4];
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const bool lo = act[0] != 0ull;
+        const unsigned long long mm = lo ? act[0] : act[1];
+        const uint32_t i = mm ? (uint32_t)__builtin_ctzll(mm) + (lo ? 0u : 64u) : 0u;
+        jj[e] = sbase + (c0 + (i >> 2)) * 256 + (i & 3);
+        if (mm) {
+          if (lo) act[0] &= act[0] - 1;
+          else act[1] &= act[1] - 1;
+          cnt = e + 1;
+        }
+      }
+      float dp[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) dp[e] = ((uint32_t)e < cnt) ? a.slots[jj[e]].depol : 0.0f;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if ((uint32_t)e < cnt) {
+          const float inc = a.dt * dp[e];
+          V = (float)((double)V + ((double)inc * a.c0) * a.e);
+        }
+      }
+      n_active += cnt;
+    }
+    // Latch releases (:697) and, on the first step after an arrival at s-1,
+    // the arrival's STDP (Synapse::run, :718-726): lastSpikeArrival = s-1,
+    // synapse trace powf(decay, 0) == 1 -> 0, target trace of lastFire as it
+    // stood after step s-1.
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      unsigned long long mm = rel[h];
+      while (mm) {
+        const uint32_t i = (uint32_t)__builtin_ctzll(mm) + 64u * h;
+        mm &= mm - 1;
+        a.stamp[sbase + (c0 + (i >> 2)) * 256 + (i & 3)] = 0;
+      }
+    }
+    while (first[0] | first[1]) {
+      uint32_t jj[2];
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        const bool lo = first[0] != 0ull;
+        const unsigned long long mm = lo ? first[0] : first[1];
+        const uint32_t i = mm ? (uint32_t)__builtin_ctzll(mm) + (lo ? 0u : 64u) : 0u;
+        jj[e] = sbase + (c0 + (i >> 2)) * 256 + (i & 3);
+        if (mm) {
+          if (lo) first[0] &= first[0] - 1;
+          else first[1] &= first[1] - 1;
+          cnt = e + 1;
+        }
+      }
+      nc_slot r[2];
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
+#pragma unroll
+      for (int e = 0; e < 2; ++e) {
+        if ((uint32_t)e < cnt && r[e].last_arr != s - 1) {
+          const float tr_arr = nc_lookup(tpost, a.len_post, (s - 1) - lf);
+          a.slots[jj[e]].w = nc_stdp(r[e].w, r[e].flags, 0.0f, tr_arr, pre_f, post_f, lr);
+          a.slots[jj[e]].last_arr = s - 1;
           ++n_arr;
         }
-        if (o >= a.r_rel) a.stamp[j] = 0;  // latch release (:697)
       }
     }
   }
@@ -206,20 +355,18 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   // charge_passive (src/NeuCor.cpp:677-680).
   V = (V - NC_BASELEVEL) * a.P + NC_BASELEVEL;
 
-  // Forced fires: device drive, background, host events.
+  // Forced fires: group spike trains, background, host events.
   bool f_in = false, f_bg = false;
   if (valid && (dw & NC_DEG_INPUT)) {
-    const nc_group g = a.groups[a.in_group[pos]];
-    const uint64_t seed = ((uint64_t)nd->drive_hi << 32) | nd->drive_lo;
-    const uint32_t u = nc_draw(seed, (uint32_t)s, g.local, NC_STREAM_DRIVE).v[0];
-    f_in = u < a.thr[g.tbl_off + (uint32_t)s % g.period];
+    const uint32_t g = nw.y;
+    f_in = ((a.gfire[(size_t)a.chunk_i * a.gwords + (g >> 5)] >> (g & 31)) & 1u) != 0;
   }
-  if (valid && nd->bg_thr) {
-    const uint64_t seed = ((uint64_t)nd->bg_hi << 32) | nd->bg_lo;
+  if (valid && nd.bg_thr) {
+    const uint64_t seed = ((uint64_t)nd.bg_hi << 32) | nd.bg_lo;
     const uint32_t u = nc_draw(seed, (uint32_t)s, a.nid[pos], NC_STREAM_BG).v[0];
-    f_bg = u < nd->bg_thr;
+    f_bg = u < nd.bg_thr;
   }
-  if (a.use_forced) {
+  if (a.use_forced && live) {
     const uint32_t wi = a.forced_in[pos >> 5], wb = a.forced_bg[pos >> 5];
     f_in |= ((wi >> (pos & 31)) & 1u) != 0;
     f_bg |= ((wb >> (pos & 31)) & 1u) != 0;
@@ -232,22 +379,60 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   // charge_thresholdCheck (src/NeuCor.cpp:682-686) + input fires (:326-331).
   const bool refr_ok = (s - lf) > a.d_ref;
   const bool fire = valid && ((((NC_THRESHOLD < V) || f_bg) && refr_ok) || f_in);
-  uint32_t fcount = 0;
   if (fire) {
     lf = s;
-    fcount = a.firings[pos] + 1u;
-    a.firings[pos] = fcount;
+    a.firings[pos] = a.firings[pos] + 1u;
   }
   // AP waveform (src/NeuCor.cpp:706-714) from the host table.
   const int32_t dlf = s - lf;
-  if (dlf <= a.d_ref) V = a.w_ap[dlf];
+  if (dlf <= a.d_ref) {
+    if (a.d_ref < NC_WAP_K) {
+      float w = a.w_ap_k[0];
+#pragma unroll
+      for (int k = 1; k < NC_WAP_K; ++k) w = (dlf == k) ? a.w_ap_k[k] : w;
+      V = w;
+    } else {
+      V = a.w_ap[dlf];
+    }
+  }
 
   if (valid) {
     a.V[pos] = V;
     if (fire) a.last_fire[pos] = lf;
   }
-  const uint32_t at = nc_wave_append(fire, a.fired_cnt);
-  if (fire) a.fired[at] = pos;
+  uint32_t n_post = 0;
+  const unsigned long long fm = __ballot(fire);
+  if (wave_in && lane == 0) a.fmask[wave] = live ? fm : 0ull;
+  if (fm) {
+    // Post-side STDP of each fired lane (the inSynapses loop of Neuron::fire,
+    // src/NeuCor.cpp:651-653): target trace powf(decay, 0) == 1 -> 0, synapse
+    // trace of lastSpikeArrival from before this step's arrivals.  The whole
+    // wave works on one fired lane's row at a time; the fence makes the row
+    // writes of the pass above visible to the other lanes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    unsigned long long m = fm;
+    while (m) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      const uint32_t dl = (uint32_t)__shfl((int)deg, (int)l);
+      const uint32_t base = rec.slot_off + l * 4;
+      n_post += (lane < dl) ? ((dl - lane + 63) >> 6) : 0u;
+      for (uint32_t k = lane; k < dl; k += 128) {
+        const bool two = k + 64 < dl;
+        const uint32_t j0 = base + (k >> 2) * 256 + (k & 3);
+        const uint32_t j1 = base + ((k + 64) >> 2) * 256 + ((k + 64) & 3);
+        const nc_slot s0 = a.slots[j0];
+        nc_slot s1 = s0;
+        if (two) s1 = a.slots[j1];
+        a.slots[j0].w = nc_stdp(s0.w, s0.flags, nc_lookup(tpre, a.len_pre, s - s0.last_arr), 0.0f, pre_f,
+                                post_f, lr);
+        if (two)
+          a.slots[j1].w = nc_stdp(s1.w, s1.flags, nc_lookup(tpre, a.len_pre, s - s1.last_arr), 0.0f, pre_f,
+                                  post_f, lr);
+      }
+    }
+  }
   if (a.raster_cap) {
     const uint32_t rt = nc_wave_append(fire, a.raster_cnt);
     if (fire && rt < a.raster_cap) {
@@ -256,66 +441,51 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
     }
   }
   if (a.ctr) {
-    nc_count(a.ctr, 0, valid ? 1ull : 0ull);
-    nc_count(a.ctr, 1, valid ? deg : 0ull);
-    nc_count(a.ctr, 2, n_active);
-    nc_count(a.ctr, 3, n_arr);
-    nc_count(a.ctr, 4, fire ? 1ull : 0ull);
-    nc_count(a.ctr, 5, fire ? deg : 0ull);
-  }
-}
-
-// Post-side STDP of every fired neuron over its incoming synapses: the
-// inSynapses loop of Neuron::fire (src/NeuCor.cpp:651-653) with the target
-// trace at powf(decay, 0) == 1 -> 0 and the synapse trace of lastSpikeArrival
-// from before this step's arrivals.
-__global__ __launch_bounds__(256) void k_post(nc_step_args a) {
-  const uint32_t nf = *a.fired_cnt;
-  const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-  for (uint32_t f = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; f < nf; f += nw) {
-    const uint32_t pos = a.fired[f];
-    const uint32_t q = pos >> 6, l = pos & 63;
-    const nc_netdesc* nd = &a.nets[a.slice_net[q]];
-    const int32_t s = nd->base_step + a.chunk_i + 1;
-    const float lr = nd->lr, pre_f = nd->pre_f, post_f = nd->post_f;
-    const uint32_t deg = a.degw[pos] & NC_DEG_MASK;
-    const uint32_t base = a.slice_off[q] + l * 4;
-    for (uint32_t k = lane; k < deg; k += 64) {
-      const uint32_t j = base + (k >> 2) * 256 + (k & 3);
-      const nc_slot sl = a.slots[j];
-      const float tr_s = nc_lookup(a.t_pre, a.len_pre, s - sl.last_arr);
-      a.slots[j].w = nc_stdp(sl.w, sl.flags, tr_s, 0.0f, pre_f, post_f, lr);
-    }
+    nc_count32(a.ctr, 2, n_active);
+    nc_count32(a.ctr, 3, n_arr);
+    if (lane == 0 && fm) atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
+    nc_count32(a.ctr, 5, n_post);
   }
 }
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
-// fired neuron: one writer per synapse (its source), latch permitting.
+// fired neuron: one wave per active slice (most leave after one load), one
+// writer per synapse (its source), latch permitting.  Accepted spikes join
+// their target slice's pending list.
 __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
-  const uint32_t nf = *a.fired_cnt;
+  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nw = (gridDim.x * blockDim.x) >> 6;
-  unsigned long long n_out = 0, n_acc = 0;
-  for (uint32_t f = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; f < nf; f += nw) {
-    const uint32_t pos = a.fired[f];
-    const nc_netdesc* nd = &a.nets[a.slice_net[pos >> 6]];
+  uint32_t n_out = 0, n_acc = 0;
+  unsigned long long m = wave < a.n_active_slices ? a.fmask[wave] : 0ull;
+  if (m) {
+    const nc_slice_rec rec = a.active_slices[wave];
+    const nc_netdesc* nd = &a.nets[rec.net];
     const int32_t s = nd->base_step + a.chunk_i + 1;
-    const uint32_t beg = a.out_beg[pos], cnt = a.out_cnt[pos];
-    n_out += (lane < cnt) ? ((cnt - lane + 63) >> 6) : 0;
-    for (uint32_t k = lane; k < cnt; k += 64) {
-      const uint32_t j = a.out_slot[beg + k];
-      if (a.stamp[j] != 0) continue;  // a spike is already in flight (:728)
-      const float w = a.slots[j].w;
-      a.slots[j].depol = a.depol_k * w;
-      a.stamp[j] = (uint8_t)nc_stamp_code(s + (int32_t)a.out_delay[beg + k]);
-      ++n_acc;
+    while (m) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(m);
+      m &= m - 1;
+      const uint32_t pos = rec.q * 64 + l;
+      const uint32_t beg = a.out_beg[pos], cnt = a.out_cnt[pos];
+      n_out += (lane < cnt) ? ((cnt - lane + 63) >> 6) : 0u;
+      for (uint32_t k = lane; k < cnt; k += 64) {
+        const uint32_t e = beg + k;
+        // a spike still in flight or in its release window (:728)
+        if (a.out_arr[e] + a.r_rel > s) continue;
+        const uint32_t j = a.out_slot[e];
+        const uint32_t d = a.out_delay[e];
+        const uint32_t tq = a.out_tq[e];
+        const float w = a.slots[j].w;
+        a.slots[j].depol = a.depol_k * w;
+        a.out_arr[e] = s + (int32_t)d;
+        const uint32_t at = atomicAdd(&a.pend_cnt[tq], 1u);
+        a.pend[a.slice_off[tq] + at] = make_uint2(j, (uint32_t)(s + (int32_t)d));
+        ++n_acc;
+      }
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) *a.fired_cnt_next = 0u;
   if (a.ctr) {
-    nc_count(a.ctr, 6, n_out);
-    nc_count(a.ctr, 7, n_acc);
+    nc_count32(a.ctr, 6, n_out);
+    nc_count32(a.ctr, 7, n_acc);
   }
 }
 
@@ -328,7 +498,7 @@ __global__ __launch_bounds__(256) void k_flush(nc_step_args a, uint32_t q0, uint
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t q = q0 + wave;
   const uint32_t pos = q * 64 + lane;
-  const uint32_t deg = a.degw[pos] & NC_DEG_MASK;
+  const uint32_t deg = a.ndw[pos].x & NC_DEG_MASK;
   const int32_t lf = a.last_fire[pos];
   const int32_t smod = s_done % 255;
   const uint32_t sbase = a.slice_off[q] + lane * 4;
