@@ -38,27 +38,15 @@
 #define NC_NEVER (-(1 << 30))
 #define NC_MAX_STEPS (1 << 29)
 
-// ---- stamp codes -----------------------------------------------------------
-// A synapse carries at most one spike (src/NeuCor.cpp:728).  Once the spike
-// has arrived, and until its latch releases, its state is one byte: 0 = idle,
-// else (arrival_step mod 255) + 1; the offset o = s - arrival is recovered
-// modulo 255, unambiguous while the release window is shorter than 255
-// steps.  Spikes still in flight wait in their target slice's pending list.
-NC_HD uint32_t nc_stamp_code(int32_t arrival_step) {
-  return (uint32_t)(arrival_step % 255) + 1u;
-}
-// smod = s % 255; returns s - arrival in [0, 254]
-NC_HD int32_t nc_stamp_offset(int32_t smod, uint32_t code) {
-  int32_t o = smod - (int32_t)code + 1;
-  return o < 0 ? o + 255 : o;
-}
-
 // ---- per-slot record (16 B, one per incoming synapse) --------------------
+// A synapse carries at most one spike (src/NeuCor.cpp:728): its record holds
+// the depolarisation and arrival step latched when the spike was accepted.
 typedef struct __attribute__((aligned(16))) nc_slot {
   float   w;        // Synapse::weight
   float   depol;    // Synapse::AP_depolFac latched at fire time
   int32_t last_arr; // Synapse::lastSpikeArrival as a step (NC_NEVER = -inf)
-  uint32_t flags;   // bit 0: inhibitory (sign class of the initial weight)
+  int32_t arr_inh;  // 2 * arrival step of the latched spike (NC_NEVER: none)
+                    // + 1 if inhibitory (sign class of the initial weight)
 } nc_slot;
 
 // ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw: "Parallel random numbers:

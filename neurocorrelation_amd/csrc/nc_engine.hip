@@ -6,12 +6,17 @@
 //   by in-degree (SELL-C-sigma, C = 64, sigma = 1024) and cut into slices of
 //   64 lanes, one wave per slice.  The incoming synapses of a slice are
 //   stored chunk-major: chunk c of lane l holds the lane's slots 4c..4c+3 at
-//   slot index slice_off + c*256 + l*4, so a wave reads 256 contiguous stamp
-//   bytes per chunk.  Slot order inside a lane is ascending source id.
-//   stamp  : u8 per slot (spike latch + arrival step code)
-//   slots  : 16 B per slot {w, depol, last_arr, flags}
-//   out-CSR: per position a row of slot indices (+ u8 delays) of the
-//            synapses it feeds, for the fan-out.
+//   slot index slice_off + c*256 + l*4, so a fired lane's row is read by the
+//   wave as 16 full 64-byte records per 64 slots.  Slot order inside a lane
+//   is ascending source id.
+//   slots  : 16 B per slot {w, depol, last_arr, 2*arrival + inhibitory}
+//   amask  : 1 bit per slot, set while the slot's spike is in its release
+//            window; words [w][lane] per slice, so a wave reads 256
+//            contiguous bytes per 32 slots of each lane
+//   pend   : per slice, (slot, arrival) of the spikes still in flight
+//   out-CSR: per position a row of slot indices, u8 delays, target slices
+//            and the source-side copy of each synapse's latch, for the
+//            fan-out.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -89,7 +94,8 @@ struct Layout {
   uint32_t n = 0, nq = 0;
   std::vector<uint32_t> pos_of_id, id_of_pos;  // id_of_pos: nq*64 (~0u = padding)
   std::vector<uint32_t> slice_slots;           // slots per slice (multiple of 256)
-  uint64_t nslots = 0;
+  std::vector<uint32_t> slice_mwords;          // active-mask words per slice (multiple of 64)
+  uint64_t nslots = 0, nmwords = 0;
 };
 
 void make_layout(const nc_net& net, Layout* L) {
@@ -123,6 +129,8 @@ void make_layout(const nc_net& net, Layout* L) {
     uint32_t nch = (mx + 3) / 4;
     L->slice_slots[q] = nch * 256;
     L->nslots += L->slice_slots[q];
+    L->slice_mwords.push_back(((mx + 31) / 32) * 64);
+    L->nmwords += L->slice_mwords.back();
   }
 }
 
@@ -190,7 +198,8 @@ struct NetRec {
   uint32_t n = 0;
   uint64_t S = 0;
   uint32_t q0 = 0, nq = 0;
-  uint64_t slot0 = 0, nslots = 0, out0 = 0;
+  uint64_t slot0 = 0, nslots = 0, out0 = 0, mask0 = 0, nmwords = 0;
+  uint32_t act0 = 0;  // first record of this network in the active-slice list
   std::vector<uint32_t> pos_of_id;    // global positions
   std::vector<uint32_t> slot_of_syn;  // global slot per synapse, export order
   std::vector<uint32_t> id_of_pos;    // local, nq*64
@@ -215,7 +224,8 @@ struct nc_sim {
   hipStream_t stream = nullptr;
   uint64_t max_pos = 0, max_slots = 0, max_out = 0;
   uint32_t max_nets = 0, raster_cap = 0;
-  RangeAlloc a_slices, a_slots, a_out;
+  RangeAlloc a_slices, a_slots, a_out, a_masks;
+  uint64_t max_mwords = 0;
   std::vector<NetRec> nets;
   bool tables_set = false;
   Tables T;
@@ -226,8 +236,8 @@ struct nc_sim {
   std::vector<hipEvent_t> ev_pool;
 
   // device buffers
-  uint8_t* stamp = nullptr;
   nc_slot* slots = nullptr;
+  uint32_t* amask = nullptr;
   float* V = nullptr;
   int32_t* last_fire = nullptr;
   uint32_t *firings = nullptr, *nid = nullptr;
@@ -257,7 +267,7 @@ struct nc_sim {
   std::vector<nc_group> groups;
   std::vector<uint32_t> thr;
   std::vector<nc_slice_rec> active_host;
-  std::vector<uint32_t> slice_off_host;
+  std::vector<uint32_t> slice_off_host, slice_moff_host;
   bool active_dirty = true, groups_dirty = true;
   std::vector<uint32_t> slice_net_host;
   uint64_t raster_read = 0;
@@ -296,15 +306,14 @@ nc_status grow(DevBuf<T>* b, size_t n, hipStream_t st) {
 
 nc_step_args base_args(nc_sim* sim) {
   nc_step_args a{};
-  a.stamp = sim->stamp;
   a.slots = sim->slots;
+  a.amask = sim->amask;
   a.V = sim->V;
   a.last_fire = sim->last_fire;
   a.firings = sim->firings;
   a.ndw = sim->ndw;
   a.nid = sim->nid;
   a.slice_off = sim->slice_off;
-  a.slice_net = sim->slice_net;
   a.active_slices = sim->active;
   a.n_active_slices = (uint32_t)sim->active_host.size();
   a.nets = sim->d_nets;
@@ -343,12 +352,38 @@ nc_step_args base_args(nc_sim* sim) {
   return a;
 }
 
+// The list of slices the step kernels visit: every slice of every live
+// network, networks in id order (each network's slices are contiguous).
+nc_status rebuild_active(nc_sim* sim) {
+  if (!sim->active_dirty) return NC_OK;
+  if (sim->upload_pending) {
+    NC_HIP(hipEventSynchronize(sim->upload_done));
+    sim->upload_pending = false;
+  }
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  sim->active_host.clear();
+  for (size_t i = 0; i < sim->nets.size(); ++i) {
+    NetRec& r = sim->nets[i];
+    if (!r.live) continue;
+    r.act0 = (uint32_t)sim->active_host.size();
+    for (uint32_t q = r.q0; q < r.q0 + r.nq; ++q)
+      sim->active_host.push_back(nc_slice_rec{q, sim->slice_off_host[q], (uint32_t)i, sim->slice_moff_host[q]});
+  }
+  if (!sim->active_host.empty())
+    NC_HIP(hipMemcpy(sim->active, sim->active_host.data(), sim->active_host.size() * sizeof(nc_slice_rec),
+                     hipMemcpyHostToDevice));
+  sim->active_dirty = false;
+  return NC_OK;
+}
+
 // Pending arrivals of the last completed step get their STDP now.
 nc_status flush_net(nc_sim* sim, NetRec& r) {
   if (r.steps_done == 0 || r.nq == 0) return NC_OK;
+  nc_status st = rebuild_active(sim);
+  if (st != NC_OK) return st;
   nc_step_args a = base_args(sim);
   uint32_t blocks = (r.nq + 3) / 4;
-  hipLaunchKernelGGL(k_flush, dim3(blocks), dim3(256), 0, sim->stream, a, r.q0, r.nq,
+  hipLaunchKernelGGL(k_flush, dim3(blocks), dim3(256), 0, sim->stream, a, r.act0, r.nq,
                      (int32_t)r.steps_done, r.lr, r.pre_f, r.post_f);
   NC_HIP(hipGetLastError());
   return NC_OK;
@@ -414,7 +449,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
   sim->raster_cap = spec->raster_capacity;
   const uint64_t P = sim->max_pos, Q = P / 64;
   if (st == NC_OK) {
-    H(hipMalloc(&sim->stamp, sim->max_slots), "hipMalloc stamp");
+    sim->max_mwords = sim->max_slots / 32 + 64 * Q;
+    H(hipMalloc(&sim->amask, sim->max_mwords * 4), "hipMalloc amask");
     H(hipMalloc(&sim->slots, sim->max_slots * sizeof(nc_slot)), "hipMalloc slots");
     H(hipMalloc(&sim->V, P * 4), "hipMalloc V");
     H(hipMalloc(&sim->last_fire, P * 4), "hipMalloc lastFire");
@@ -443,7 +479,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMalloc(&sim->d_nets, sim->max_nets * sizeof(nc_netdesc)), "hipMalloc nets");
   }
   if (st == NC_OK) {
-    H(hipMemsetAsync(sim->stamp, 0, sim->max_slots, sim->stream), "memset");
+    H(hipMemsetAsync(sim->amask, 0, sim->max_mwords * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->ndw, 0, P * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
@@ -454,12 +490,14 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     std::vector<uint32_t> none(Q, NC_SLICE_NONE);
     sim->slice_net_host = none;
     sim->slice_off_host.assign(Q, 0u);
+    sim->slice_moff_host.assign(Q, 0u);
     H(hipMemcpyAsync(sim->slice_net, none.data(), Q * 4, hipMemcpyHostToDevice, sim->stream), "memcpy");
     H(hipStreamSynchronize(sim->stream), "sync");
   }
   sim->a_slices.init(Q);
   sim->a_slots.init(sim->max_slots / 256);
   sim->a_out.init(sim->max_out);
+  sim->a_masks.init(sim->max_mwords / 64);
   if (st != NC_OK) {
     nc_sim_destroy(sim);
     return st;
@@ -472,7 +510,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (!sim) return;
   (void)hipSetDevice(sim->device);
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
-  void* ptrs[] = {sim->stamp, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
+  void* ptrs[] = {sim->amask, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
                   sim->nid, sim->out_beg, sim->out_cnt, sim->out_slot,
                   sim->out_delay, sim->out_arr, sim->out_tq, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fmask, sim->raster, sim->raster_cnt,
@@ -551,7 +589,15 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     sim->a_slots.release(sl0, L.nslots / 256);
     return fail(NC_ERR_CAPACITY, "no room for the out-synapse lists");
   }
+  uint64_t mk0 = 0;
+  if (!sim->a_masks.alloc(L.nmwords / 64, &mk0)) {
+    sim->a_slices.release(q0, L.nq);
+    sim->a_slots.release(sl0, L.nslots / 256);
+    sim->a_out.release(o0, S);
+    return fail(NC_ERR_CAPACITY, "no room for the active-slot masks");
+  }
   sl0 *= 256;
+  mk0 *= 64;
 
   NetRec& r = sim->nets[id];
   r = NetRec();
@@ -562,6 +608,8 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   r.slot0 = sl0;
   r.nslots = L.nslots;
   r.out0 = o0;
+  r.mask0 = mk0;
+  r.nmwords = L.nmwords;
   r.lr = p.learning_rate;
   r.pre_f = p.pre_factor;
   r.post_f = p.post_factor;
@@ -572,17 +620,19 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
 
   const uint32_t npos = L.nq * 64;
   const uint32_t pos0 = (uint32_t)(q0 * 64);
-  std::vector<uint32_t> slice_off(L.nq);
+  std::vector<uint32_t> slice_off(L.nq), slice_moff(L.nq);
   {
-    uint64_t o = sl0;
+    uint64_t o = sl0, mo = mk0;
     for (uint32_t q = 0; q < L.nq; ++q) {
       slice_off[q] = (uint32_t)o;
+      slice_moff[q] = (uint32_t)mo;
       o += L.slice_slots[q];
+      mo += L.slice_mwords[q];
     }
   }
   // Slot records, in layout order.
   std::vector<nc_slot> hs(L.nslots);
-  for (auto& x : hs) x = nc_slot{0.0f, 0.0f, NC_NEVER, 0u};
+  for (auto& x : hs) x = nc_slot{0.0f, 0.0f, NC_NEVER, 2 * NC_NEVER};
   r.slot_of_syn.resize(S);
   for (uint32_t id_ = 0; id_ < net->n; ++id_) {
     const uint32_t pl = L.pos_of_id[id_];
@@ -593,7 +643,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       r.slot_of_syn[b + k] = j;
       nc_slot& x = hs[j - sl0];
       x.w = net->w[b + k];
-      x.flags = net->inh[b + k] ? 1u : 0u;
+      x.arr_inh = 2 * NC_NEVER + (net->inh[b + k] ? 1 : 0);
     }
   }
   // Out-CSR by source position; rows in ascending target id.
@@ -639,6 +689,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       sim->a_slices.release(q0, L.nq);
       sim->a_slots.release(sl0 / 256, L.nslots / 256);
       sim->a_out.release(o0, S);
+      sim->a_masks.release(mk0 / 64, L.nmwords / 64);
       r.live = false;
       return stt;
     }
@@ -654,7 +705,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       G.tbl_off = (uint32_t)sim->thr.size();
       G.period = T;
       G.local = g;
-      G.pad = (uint32_t)id;
+      G.net = (uint32_t)id;
       const double amp = g < ncorr ? drive->amp_corr : drive->amp_ind;
       for (uint32_t k = 0; k < T; ++k) {
         double pr = (double)drive->p0 * (1.0 + amp * sin(2.0 * M_PI * (double)k / (double)T));
@@ -689,7 +740,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   std::vector<uint32_t> zero(npos, 0), snet(L.nq, (uint32_t)id);
   hipStream_t st = sim->stream;
   NC_HIP(hipMemcpyAsync(sim->slots + sl0, hs.data(), L.nslots * sizeof(nc_slot), hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemsetAsync(sim->stamp + sl0, 0, L.nslots, st));
+  NC_HIP(hipMemsetAsync(sim->amask + mk0, 0, L.nmwords * 4, st));
   NC_HIP(hipMemcpyAsync(sim->V + pos0, V0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->firings + pos0, zero.data(), npos * 4, hipMemcpyHostToDevice, st));
@@ -715,6 +766,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   for (uint32_t q = 0; q < L.nq; ++q) {
     sim->slice_net_host[q0 + q] = (uint32_t)id;
     sim->slice_off_host[q0 + q] = slice_off[q];
+    sim->slice_moff_host[q0 + q] = slice_moff[q];
   }
   r.live = true;
   sim->active_dirty = true;
@@ -734,6 +786,7 @@ extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) {
   sim->a_slices.release(r->q0, r->nq);
   sim->a_slots.release(r->slot0 / 256, r->nslots / 256);
   sim->a_out.release(r->out0, r->S);
+  sim->a_masks.release(r->mask0 / 64, r->nmwords / 64);
   // Drive groups stay allocated in the host vectors (append-only); they are
   // compacted when no network uses them.
   r->live = false;
@@ -805,18 +858,9 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     max_run = std::max(max_run, run);
   }
   if (!any_active) return NC_OK;
-  if (sim->active_dirty) {
-    sim->active_host.clear();
-    for (size_t i = 0; i < sim->nets.size(); ++i) {
-      const NetRec& r = sim->nets[i];
-      if (!r.live) continue;
-      for (uint32_t q = r.q0; q < r.q0 + r.nq; ++q)
-        sim->active_host.push_back(nc_slice_rec{q, sim->slice_off_host[q], (uint32_t)i, 0u});
-    }
-    if (!sim->active_host.empty())
-      NC_HIP(hipMemcpyAsync(sim->active, sim->active_host.data(), sim->active_host.size() * sizeof(nc_slice_rec),
-                            hipMemcpyHostToDevice, st));
-    sim->active_dirty = false;
+  {
+    nc_status s0 = rebuild_active(sim);
+    if (s0 != NC_OK) return s0;
   }
   NC_HIP(hipMemcpyAsync(sim->d_nets, nds.data(), nds.size() * sizeof(nc_netdesc), hipMemcpyHostToDevice, st));
 
@@ -991,17 +1035,13 @@ nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t 
   return NC_OK;
 }
 
-nc_status read_slots(nc_sim* sim, NetRec& r, std::vector<nc_slot>* hs, std::vector<uint8_t>* hst) {
+nc_status read_slots(nc_sim* sim, NetRec& r, std::vector<nc_slot>* hs) {
   NC_HIP(hipSetDevice(sim->device));
   nc_status st = flush_net(sim, r);
   if (st != NC_OK) return st;
   hs->resize(r.nslots);
   NC_HIP(hipMemcpyAsync(hs->data(), sim->slots + r.slot0, r.nslots * sizeof(nc_slot), hipMemcpyDeviceToHost,
                         sim->stream));
-  if (hst) {
-    hst->resize(r.nslots);
-    NC_HIP(hipMemcpyAsync(hst->data(), sim->stamp + r.slot0, r.nslots, hipMemcpyDeviceToHost, sim->stream));
-  }
   NC_HIP(hipStreamSynchronize(sim->stream));
   return NC_OK;
 }
@@ -1038,7 +1078,7 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   if (!out) return fail(NC_ERR_ARG, "null argument");
   if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
   std::vector<nc_slot> hs;
-  st = read_slots(sim, *r, &hs, nullptr);
+  st = read_slots(sim, *r, &hs);
   if (st != NC_OK) return st;
   for (uint64_t k = 0; k < n; ++k) out[k] = hs[r->slot_of_syn[k] - r->slot0].w;
   return NC_OK;
@@ -1051,13 +1091,13 @@ extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* 
   if (!w) return fail(NC_ERR_ARG, "null argument");
   if (n > r->S) return fail(NC_ERR_RANGE, "more synapses given than the network has");
   std::vector<nc_slot> hs;
-  st = read_slots(sim, *r, &hs, nullptr);
+  st = read_slots(sim, *r, &hs);
   if (st != NC_OK) return st;
   // Synapse::setWeight also re-derives the sign class (src/NeuCor.cpp:576-579).
   for (uint64_t k = 0; k < n; ++k) {
     nc_slot& x = hs[r->slot_of_syn[k] - r->slot0];
     x.w = w[k];
-    x.flags = (w[k] < 0.0f) ? 1u : 0u;
+    x.arr_inh = (x.arr_inh & ~1) | ((w[k] < 0.0f) ? 1 : 0);
   }
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice,
                         sim->stream));
@@ -1072,31 +1112,15 @@ extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t*
   if (st != NC_OK) return st;
   if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
   std::vector<nc_slot> hs;
-  std::vector<uint8_t> hst;
-  st = read_slots(sim, *r, &hs, &hst);
+  st = read_slots(sim, *r, &hs);
   if (st != NC_OK) return st;
-  // spikes still in flight: the pending lists of the network's slices
-  std::vector<int32_t> inflight(r->nslots, 0);
-  {
-    std::vector<uint32_t> cnt(r->nq);
-    std::vector<uint2> ent(r->nslots);
-    NC_HIP(hipMemcpy(cnt.data(), sim->pend_cnt + r->q0, r->nq * 4, hipMemcpyDeviceToHost));
-    NC_HIP(hipMemcpy(ent.data(), sim->pend + r->slot0, r->nslots * sizeof(uint2), hipMemcpyDeviceToHost));
-    for (uint32_t q = 0; q < r->nq; ++q) {
-      const uint64_t b = sim->slice_off_host[r->q0 + q] - r->slot0;
-      for (uint32_t t = 0; t < cnt[q]; ++t) inflight[ent[b + t].x - r->slot0] = (int32_t)ent[b + t].y;
-    }
-  }
+  // Synapse::AP_fireTime: the latched spike's arrival until its latch
+  // releases (R steps after arrival), then 0.
   const int32_t s = (int32_t)r->steps_done;
-  const int32_t smod = s % 255;
   for (uint64_t k = 0; k < n; ++k) {
     const uint64_t j = r->slot_of_syn[k] - r->slot0;
-    const uint32_t code = hst[j];
-    if (arrival) {
-      if (inflight[j]) arrival[k] = inflight[j];
-      else if (code == 0) arrival[k] = 0;
-      else arrival[k] = s - nc_stamp_offset(smod, code);
-    }
+    const int32_t a = hs[j].arr_inh >> 1;
+    if (arrival) arrival[k] = (a != NC_NEVER && s < a + sim->T.r_rel) ? a : 0;
     if (last_arrival) last_arrival[k] = hs[j].last_arr;
     if (depol) depol[k] = hs[j].depol;
   }

@@ -2,21 +2,20 @@
 //
 // One step s of every active network is, in order:
 //   k_drive   host-scheduled forced fires -> bitmasks       (only when present)
-//   k_neuron  one lane per neuron (one wave per 64-neuron slice): moves the
-//             slice's spikes arriving at s from its pending list into the
-//             stamp bytes; charge_insynapses, charge_passive, threshold /
-//             forced fire, deferred arrival STDP, AP waveform, latch
-//             release; then the wave applies post-side STDP to the incoming
-//             synapses of each of its fired lanes and writes the slice's
-//             fired bitmask
-//   k_fanout  waves over the slices with fired lanes: Synapse::fire on
-//             their outgoing synapses (source-order latch copy, out->in
-//             slot map), appending accepted spikes to the target slice's
-//             pending list
-// The kernel boundary orders the phases.  No atomics in the step: every
-// synapse slot has one writer per phase (its target's lane, then its
-// source's wave), fired lists are per-slice ballots, counters are owned by
-// the block that adds to them.
+//   k_neuron  one lane per neuron (one wave per 64-neuron slice):
+//             charge_insynapses over the neuron's active-slot bits,
+//             deferred arrival STDP and latch releases on the same pass,
+//             charge_passive, threshold / forced fire, AP waveform; then the
+//             wave applies post-side STDP to the incoming synapses of each
+//             of its fired lanes, and turns the slice's spikes arriving at s
+//             (pending list) into active-slot bits
+//   k_fanout  one wave per slice with fired lanes: Synapse::fire on their
+//             outgoing synapses (source-order latch copy, out->in slot map),
+//             appending accepted spikes to the target slice's pending list
+// The kernel boundary orders the phases.  Every synapse slot has one writer
+// per phase (its target's wave in k_neuron, its source's wave in k_fanout);
+// the only atomics are the pending-list append (per target slice) and the
+// arrival bits (workgroup scope, inside one wave's own slice).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -28,6 +27,7 @@
 #define NC_SLICE_NONE 0xFFFFFFFFu
 #define NC_LDS_TAB 2048  // trace-table entries staged in LDS per block
 #define NC_WAP_K 8       // AP-table entries passed as kernel arguments
+#define NC_MW 4          // active-mask words (32 slots each) a lane holds per pass
 #define NC_NCTR 8
 
 // Per-network descriptor, rewritten by the host before each batch of steps.
@@ -41,44 +41,43 @@ struct nc_netdesc {
   uint32_t pad[6];
 };
 
-// One record per active slice (one wave): its index, first slot, network.
+// One record per active slice (one wave).
 struct nc_slice_rec {
-  uint32_t q;
-  uint32_t slot_off;
-  uint32_t net;
-  uint32_t pad;
+  uint32_t q;         // slice index (positions 64q .. 64q+63)
+  uint32_t slot_off;  // first synapse slot of the slice
+  uint32_t net;       // network id
+  uint32_t mask_off;  // first active-mask word of the slice
 };
 
 struct nc_group {
   uint32_t tbl_off;  // first threshold entry
   uint32_t period;   // entries
   uint32_t local;    // group index inside its network (Philox counter word)
-  uint32_t pad;      // network id
+  uint32_t net;      // network id
 };
 
 struct nc_step_args {
-  uint8_t* stamp;
   nc_slot* slots;
+  uint32_t* amask;  // per slice: [mask word][lane] bits of active slots
   float* V;
   int32_t* last_fire;
   uint32_t* firings;
   const uint2* ndw;  // per position: x = in-degree | flags, y = input group
   const uint32_t* nid;
   const uint32_t* slice_off;
-  const uint32_t* slice_net;
   const nc_slice_rec* active_slices;
   uint32_t n_active_slices;
   const nc_netdesc* nets;
   const nc_group* groups;
   const uint32_t* thr;
-  uint32_t* gfire;       // [chunk step][group words]: group spike trains of this batch
-  uint32_t gwords;       // words per step in gfire
+  uint32_t* gfire;  // [chunk step][group words]: group spike trains of this batch
+  uint32_t gwords;  // words per step in gfire
   uint32_t n_groups;
   uint32_t* forced_in;
   uint32_t* forced_bg;
   int32_t use_forced;
   unsigned long long* fmask;  // fired lanes of each active slice (by active-list index), this step
-  uint32_t* raster;  // (pos, step) pairs
+  uint32_t* raster;           // (pos, step) pairs
   uint32_t* raster_cnt;
   uint32_t raster_cap;
   unsigned long long* ctr;  // [grid blocks][NC_NCTR], owned per block
@@ -86,7 +85,7 @@ struct nc_step_args {
   const float* t_post;
   const float* w_ap;
   int32_t len_pre, len_post;
-  int32_t tabs_in_lds;  // both trace tables fit NC_LDS_TAB entries
+  int32_t tabs_in_lds;     // both trace tables fit NC_LDS_TAB entries
   float w_ap_k[NC_WAP_K];  // the AP table when it has <= NC_WAP_K entries
   float dt, P, depol_k;
   double c0, e;
@@ -96,8 +95,8 @@ struct nc_step_args {
   const uint32_t* out_cnt;
   const uint32_t* out_slot;
   const uint8_t* out_delay;
-  int32_t* out_arr;   // source order: arrival step of the synapse's last accepted spike
-                      // (NC_NEVER: none); its latch holds until arrival + release window
+  int32_t* out_arr;        // source order: arrival step of the synapse's last accepted spike
+                           // (NC_NEVER: none); its latch holds until arrival + release window
   const uint32_t* out_tq;  // target slice of each out-synapse
   uint2* pend;             // per slice, at its slot range: (slot, arrival step) of spikes in flight
   uint32_t* pend_cnt;      // per slice: entries in its pending list
@@ -112,28 +111,23 @@ __device__ __forceinline__ float nc_lookup(const float* t, int32_t len, int32_t 
 
 // fmax(fmin(w, hi), lo) as the reference evaluates it on x86-64
 // (src/NeuCor.cpp:760-761); exact in float because the bounds are.
-__device__ __forceinline__ float nc_clamp_w(float w, uint32_t flags) {
-  const float hi = (flags & 1u) ? 0.0f : 1.0f;
-  const float lo = (flags & 1u) ? -1.0f : 0.0f;
+__device__ __forceinline__ float nc_clamp_w(float w, uint32_t inh) {
+  const float hi = inh ? 0.0f : 1.0f;
+  const float lo = inh ? -1.0f : 0.0f;
   float m = (w != w) ? hi : (w < hi ? w : hi);
   return m > lo ? m : lo;
 }
 
 // Synapse::synapticPlasticity (src/NeuCor.cpp:740-764) with both traces given.
-__device__ __forceinline__ float nc_stdp(float w, uint32_t flags, float tr_s, float tr_t,
-                                         float pre_f, float post_f, float lr) {
+__device__ __forceinline__ float nc_stdp(float w, uint32_t inh, float tr_s, float tr_t, float pre_f,
+                                         float post_f, float lr) {
   float dw = pre_f * tr_s - post_f * tr_t;
   float wn = w + dw * lr;
-  return nc_clamp_w(wn, flags);
+  return nc_clamp_w(wn, inh);
 }
 
 __device__ __forceinline__ uint32_t nc_lane() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
-}
-
-__device__ __forceinline__ unsigned long long nc_wave_sum(unsigned long long v) {
-  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
-  return v;
 }
 
 // Wave-aggregated append of `pred` lanes to list[*cnt ...]; returns the slot
@@ -151,11 +145,16 @@ __device__ __forceinline__ uint32_t nc_wave_append(bool pred, uint32_t* cnt) {
 }
 
 // Adds per-lane counts into the block's own counter row: a wave sum, then
-// one fire-and-forget atomic per counter from lane 0 (rows are per block,
-// so they see at most 4 writers).
+// one fire-and-forget atomic from lane 0 (a row sees at most 4 writers).
 __device__ __forceinline__ void nc_count32(unsigned long long* ctr, int k, uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
   if (nc_lane() == 0 && v) atomicAdd(&ctr[(size_t)blockIdx.x * NC_NCTR + k], (unsigned long long)v);
+}
+
+// Slot index of in-slot k of lane l in a slice whose slots start at slot_off
+// (chunk-major: 4 slots per lane per chunk).
+__device__ __forceinline__ uint32_t nc_slot_of(uint32_t slot_off, uint32_t l, uint32_t k) {
+  return slot_off + (k >> 2) * 256 + l * 4 + (k & 3);
 }
 
 // Group spike trains of a batch of steps: one Philox draw per (step, group),
@@ -167,7 +166,7 @@ __global__ void k_groups(nc_step_args a, uint32_t n_steps) {
   bool fire = false;
   if (g < a.n_groups) {
     const nc_group G = a.groups[g];
-    const nc_netdesc* nd = &a.nets[G.pad];
+    const nc_netdesc* nd = &a.nets[G.net];
     if (i < (uint32_t)nd->remaining) {
       const uint32_t s = (uint32_t)(nd->base_step + (int32_t)i + 1);
       const uint64_t seed = ((uint64_t)nd->drive_hi << 32) | nd->drive_lo;
@@ -190,10 +189,8 @@ __global__ void k_drive(nc_step_args a) {
   atomicOr(&m[pos >> 5], 1u << (pos & 31));
 }
 
-#define NC_BATCH 8  // stamp words (4 slots each) loaded per lane before use: 32 slots
-
 // Neuron::run (src/NeuCor.cpp:619-641) for every neuron of the active slices.
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k_neuron(nc_step_args a) {
+__global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   __shared__ float sh_pre[NC_LDS_TAB], sh_post[NC_LDS_TAB];
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t lane = threadIdx.x & 63;
@@ -219,137 +216,68 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   __syncthreads();
   const float* tpre = a.tabs_in_lds ? sh_pre : a.t_pre;
   const float* tpost = a.tabs_in_lds ? sh_post : a.t_post;
-
-  const int32_t smod = s % 255;
-  const uint32_t sbase = rec.slot_off + lane * 4;
-  const uint32_t nch = (deg + 3) >> 2;
   uint32_t n_active = 0, n_arr = 0;
 
-  // Spikes arriving now leave the pending list for the stamp bytes; the
-  // list is compacted in place (write index <= read index).
-  if (pend_n) {
-    uint2* pl = a.pend + rec.slot_off;
-    uint32_t kept = 0;
-    for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
-      const uint32_t t = t0 + lane;
-      uint2 e = make_uint2(0u, 0u);
-      if (t < pend_n) e = pl[t];
-      const bool arrive = t < pend_n && (int32_t)e.y == s;
-      if (arrive) a.stamp[e.x] = (uint8_t)nc_stamp_code(s);
-      const bool keep = t < pend_n && !arrive;
-      const unsigned long long km = __ballot(keep);
-      if (keep) pl[kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull))] = e;
-      kept += (uint32_t)__builtin_popcountll(km);
-    }
-    if (lane == 0) a.pend_cnt[q] = kept;
-  }
-
-  // charge_insynapses (src/NeuCor.cpp:688-700) in ascending source order,
-  // in groups of 4 x NC_BATCH stamp words (128 slots): the group's words are
-  // loaded NC_BATCH at a time and reduced to masks; then the active slots of
-  // the whole group are handled.
-  for (uint32_t c0 = 0; c0 < nch; c0 += 4 * NC_BATCH) {
-    // bit i of act/first/rel[h]: slot 64*h + i of the group is contributing /
-    // on its first step after arrival / on its last step before release
-    unsigned long long act[2] = {0ull, 0ull}, first[2] = {0ull, 0ull}, rel[2] = {0ull, 0ull};
+  // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
+  // that arrived at s-R .. s-1) in ascending source order.  Each slot record
+  // carries the latched depolarisation and the arrival step, so one fetch
+  // serves the contribution, the arrival STDP and the latch release.
+  const uint32_t nmw = (deg + 31) >> 5;
+  uint32_t* mrow = a.amask + rec.mask_off + lane;
+  for (uint32_t w0 = 0; w0 < nmw; w0 += NC_MW) {
+    uint32_t m[NC_MW], m0[NC_MW];
 #pragma unroll
-    for (int bt = 0; bt < 4; ++bt) {
-      uint32_t wv[NC_BATCH];
+    for (int u = 0; u < NC_MW; ++u) m[u] = m0[u] = (w0 + u < nmw) ? mrow[(w0 + u) * 64] : 0u;
 #pragma unroll
-      for (int u = 0; u < NC_BATCH; ++u) {
-        const uint32_t c = c0 + bt * NC_BATCH + u;
-        wv[u] = (c < nch) ? *(const uint32_t*)(a.stamp + sbase + c * 256) : 0u;
-      }
-      uint32_t am = 0, fm = 0, rm = 0;
+    for (int u = 0; u < NC_MW; ++u) {
+      uint32_t bits = m[u];
+      while (bits) {
+        // up to four records fetched together, applied in slot order
+        uint32_t jj[4];
+        uint32_t cnt = 0;
 #pragma unroll
-      for (int u = 0; u < NC_BATCH; ++u) {
-        const uint32_t w4 = wv[u];
-        if (w4 == 0u) continue;
-#pragma unroll
-        for (int b = 0; b < 4; ++b) {
-          const uint32_t code = (w4 >> (8 * b)) & 0xffu;
-          const int32_t o = nc_stamp_offset(smod, code);
-          const uint32_t bit = (uint32_t)(code != 0u && o >= 1 && o <= a.r_rel) << (4 * u + b);
-          am |= bit;
-          fm |= (o == 1) ? bit : 0u;
-          rm |= (o >= a.r_rel) ? bit : 0u;
+        for (int e = 0; e < 4; ++e) {
+          const uint32_t k = 32u * (w0 + u) + (bits ? (uint32_t)__builtin_ctz(bits) : 0u);
+          jj[e] = nc_slot_of(rec.slot_off, lane, k);
+          if (bits) {
+            bits &= bits - 1;
+            cnt = e + 1;
+          }
         }
-      }
-      act[bt >> 1] |= (unsigned long long)am << (32 * (bt & 1));
-      first[bt >> 1] |= (unsigned long long)fm << (32 * (bt & 1));
-      rel[bt >> 1] |= (unsigned long long)rm << (32 * (bt & 1));
-    }
-    // Contributions: the latched depolarisation of up to four active slots is
-    // fetched at once, then added in slot order.
-    while (act[0] | act[1]) {
-      uint32_t jj[4];
-      uint32_t cnt = 0;
+        nc_slot r[4];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const bool lo = act[0] != 0ull;
-        const unsigned long long mm = lo ? act[0] : act[1];
-        const uint32_t i = mm ? (uint32_t)__builtin_ctzll(mm) + (lo ? 0u : 64u) : 0u;
-        jj[e] = sbase + (c0 + (i >> 2)) * 256 + (i & 3);
-        if (mm) {
-          if (lo) act[0] &= act[0] - 1;
-          else act[1] &= act[1] - 1;
-          cnt = e + 1;
-        }
-      }
-      float dp[4];
+        for (int e = 0; e < 4; ++e)
+          if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) dp[e] = ((uint32_t)e < cnt) ? a.slots[jj[e]].depol : 0.0f;
-#pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        if ((uint32_t)e < cnt) {
-          const float inc = a.dt * dp[e];
+        for (int e = 0; e < 4; ++e) {
+          if ((uint32_t)e >= cnt) break;
+          const int32_t arr = r[e].arr_inh >> 1;
+          const int32_t o = s - arr;  // in [1, R]: the bit is cleared at o == R
+          const float inc = a.dt * r[e].depol;
           V = (float)((double)V + ((double)inc * a.c0) * a.e);
+          if (o == 1 && r[e].last_arr != arr) {
+            // the arrival of step s-1 (Synapse::run, :718-726):
+            // lastSpikeArrival = s-1, synapse trace powf(decay, 0) == 1 -> 0,
+            // target trace of lastFire as it stood after step s-1
+            const float tr_arr = nc_lookup(tpost, a.len_post, arr - lf);
+            a.slots[jj[e]].w = nc_stdp(r[e].w, (uint32_t)r[e].arr_inh & 1u, 0.0f, tr_arr, pre_f, post_f, lr);
+            a.slots[jj[e]].last_arr = arr;
+            ++n_arr;
+          }
+          if (o >= a.r_rel) {  // latch release (:697)
+            const uint32_t rel = jj[e] - rec.slot_off;
+            const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
+#pragma unroll
+            for (int u2 = 0; u2 < NC_MW; ++u2)
+              if (w0 + u2 == (k >> 5)) m[u2] &= ~(1u << (k & 31));
+          }
         }
-      }
-      n_active += cnt;
-    }
-    // Latch releases (:697) and, on the first step after an arrival at s-1,
-    // the arrival's STDP (Synapse::run, :718-726): lastSpikeArrival = s-1,
-    // synapse trace powf(decay, 0) == 1 -> 0, target trace of lastFire as it
-    // stood after step s-1.
-#pragma unroll
-    for (int h = 0; h < 2; ++h) {
-      unsigned long long mm = rel[h];
-      while (mm) {
-        const uint32_t i = (uint32_t)__builtin_ctzll(mm) + 64u * h;
-        mm &= mm - 1;
-        a.stamp[sbase + (c0 + (i >> 2)) * 256 + (i & 3)] = 0;
-      }
-    }
-    while (first[0] | first[1]) {
-      uint32_t jj[2];
-      uint32_t cnt = 0;
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        const bool lo = first[0] != 0ull;
-        const unsigned long long mm = lo ? first[0] : first[1];
-        const uint32_t i = mm ? (uint32_t)__builtin_ctzll(mm) + (lo ? 0u : 64u) : 0u;
-        jj[e] = sbase + (c0 + (i >> 2)) * 256 + (i & 3);
-        if (mm) {
-          if (lo) first[0] &= first[0] - 1;
-          else first[1] &= first[1] - 1;
-          cnt = e + 1;
-        }
-      }
-      nc_slot r[2];
-#pragma unroll
-      for (int e = 0; e < 2; ++e)
-        if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
-#pragma unroll
-      for (int e = 0; e < 2; ++e) {
-        if ((uint32_t)e < cnt && r[e].last_arr != s - 1) {
-          const float tr_arr = nc_lookup(tpost, a.len_post, (s - 1) - lf);
-          a.slots[jj[e]].w = nc_stdp(r[e].w, r[e].flags, 0.0f, tr_arr, pre_f, post_f, lr);
-          a.slots[jj[e]].last_arr = s - 1;
-          ++n_arr;
-        }
+        n_active += cnt;
       }
     }
+#pragma unroll
+    for (int u = 0; u < NC_MW; ++u)
+      if (m[u] != m0[u]) mrow[(w0 + u) * 64] = m[u];
   }
 
   // charge_passive (src/NeuCor.cpp:677-680).
@@ -387,10 +315,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   const int32_t dlf = s - lf;
   if (dlf <= a.d_ref) {
     if (a.d_ref < NC_WAP_K) {
-      float w = a.w_ap_k[0];
+      float wv = a.w_ap_k[0];
 #pragma unroll
-      for (int k = 1; k < NC_WAP_K; ++k) w = (dlf == k) ? a.w_ap_k[k] : w;
-      V = w;
+      for (int k = 1; k < NC_WAP_K; ++k) wv = (dlf == k) ? a.w_ap_k[k] : wv;
+      V = wv;
     } else {
       V = a.w_ap[dlf];
     }
@@ -403,36 +331,65 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
   if (wave_in && lane == 0) a.fmask[wave] = live ? fm : 0ull;
+  // The passes below touch slot records and mask words this wave wrote above,
+  // from other lanes: make those writes visible first.
+  if (fm || pend_n) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  }
   if (fm) {
     // Post-side STDP of each fired lane (the inSynapses loop of Neuron::fire,
     // src/NeuCor.cpp:651-653): target trace powf(decay, 0) == 1 -> 0, synapse
     // trace of lastSpikeArrival from before this step's arrivals.  The whole
-    // wave works on one fired lane's row at a time; the fence makes the row
-    // writes of the pass above visible to the other lanes.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    unsigned long long m = fm;
-    while (m) {
-      const uint32_t l = (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
+    // wave works on one fired lane's row at a time.
+    unsigned long long mm = fm;
+    while (mm) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(mm);
+      mm &= mm - 1;
       const uint32_t dl = (uint32_t)__shfl((int)deg, (int)l);
-      const uint32_t base = rec.slot_off + l * 4;
       n_post += (lane < dl) ? ((dl - lane + 63) >> 6) : 0u;
       for (uint32_t k = lane; k < dl; k += 128) {
         const bool two = k + 64 < dl;
-        const uint32_t j0 = base + (k >> 2) * 256 + (k & 3);
-        const uint32_t j1 = base + ((k + 64) >> 2) * 256 + ((k + 64) & 3);
+        const uint32_t j0 = nc_slot_of(rec.slot_off, l, k);
+        const uint32_t j1 = nc_slot_of(rec.slot_off, l, k + 64);
         const nc_slot s0 = a.slots[j0];
         nc_slot s1 = s0;
         if (two) s1 = a.slots[j1];
-        a.slots[j0].w = nc_stdp(s0.w, s0.flags, nc_lookup(tpre, a.len_pre, s - s0.last_arr), 0.0f, pre_f,
-                                post_f, lr);
+        a.slots[j0].w = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - s0.last_arr),
+                                0.0f, pre_f, post_f, lr);
         if (two)
-          a.slots[j1].w = nc_stdp(s1.w, s1.flags, nc_lookup(tpre, a.len_pre, s - s1.last_arr), 0.0f, pre_f,
-                                  post_f, lr);
+          a.slots[j1].w = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - s1.last_arr),
+                                  0.0f, pre_f, post_f, lr);
       }
     }
   }
+
+  // Spikes of this slice arriving at s leave the pending list and set their
+  // slot's active bit (contributing from s+1).  The list is compacted in
+  // place (write index <= read index).
+  if (pend_n) {
+    uint2* pl = a.pend + rec.slot_off;
+    uint32_t kept = 0;
+    for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      uint2 e = make_uint2(0u, 0u);
+      if (t < pend_n) e = pl[t];
+      const bool arrive = t < pend_n && (int32_t)e.y == s;
+      if (arrive) {
+        const uint32_t rel = e.x - rec.slot_off;
+        const uint32_t tl = (rel & 255u) >> 2;
+        const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
+        __hip_atomic_fetch_or(a.amask + rec.mask_off + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      const bool keep = t < pend_n && !arrive;
+      const unsigned long long km = __ballot(keep);
+      if (keep) pl[kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull))] = e;
+      kept += (uint32_t)__builtin_popcountll(km);
+    }
+    if (lane == 0) a.pend_cnt[q] = kept;
+  }
+
   if (a.raster_cap) {
     const uint32_t rt = nc_wave_append(fire, a.raster_cnt);
     if (fire && rt < a.raster_cap) {
@@ -443,15 +400,17 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void k
   if (a.ctr) {
     nc_count32(a.ctr, 2, n_active);
     nc_count32(a.ctr, 3, n_arr);
-    if (lane == 0 && fm) atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
+    if (lane == 0 && fm)
+      atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
     nc_count32(a.ctr, 5, n_post);
   }
 }
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
 // fired neuron: one wave per active slice (most leave after one load), one
-// writer per synapse (its source), latch permitting.  Accepted spikes join
-// their target slice's pending list.
+// writer per synapse (its source), latch permitting.  Accepted spikes latch
+// depolarisation and arrival step into the target slot and join the target
+// slice's pending list.
 __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
@@ -472,13 +431,14 @@ __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
         // a spike still in flight or in its release window (:728)
         if (a.out_arr[e] + a.r_rel > s) continue;
         const uint32_t j = a.out_slot[e];
-        const uint32_t d = a.out_delay[e];
+        const int32_t arr = s + (int32_t)a.out_delay[e];
         const uint32_t tq = a.out_tq[e];
-        const float w = a.slots[j].w;
-        a.slots[j].depol = a.depol_k * w;
-        a.out_arr[e] = s + (int32_t)d;
+        const nc_slot r = a.slots[j];
+        a.slots[j].depol = a.depol_k * r.w;
+        a.slots[j].arr_inh = arr * 2 + (r.arr_inh & 1);
+        a.out_arr[e] = arr;
         const uint32_t at = atomicAdd(&a.pend_cnt[tq], 1u);
-        a.pend[a.slice_off[tq] + at] = make_uint2(j, (uint32_t)(s + (int32_t)d));
+        a.pend[a.slice_off[tq] + at] = make_uint2(j, (uint32_t)arr);
         ++n_acc;
       }
     }
@@ -491,25 +451,27 @@ __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
 
 // Applies the STDP of arrivals at step s_done that k_neuron defers to the
 // next step, so that host reads and parameter changes see eager state.
-__global__ __launch_bounds__(256) void k_flush(nc_step_args a, uint32_t q0, uint32_t nq,
-                                               int32_t s_done, float lr, float pre_f, float post_f) {
+__global__ __launch_bounds__(256) void k_flush(nc_step_args a, uint32_t first_rec, uint32_t n_rec, int32_t s_done,
+                                               float lr, float pre_f, float post_f) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (wave >= nq) return;
+  if (wave >= n_rec) return;
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t q = q0 + wave;
-  const uint32_t pos = q * 64 + lane;
+  const nc_slice_rec rec = a.active_slices[first_rec + wave];
+  const uint32_t pos = rec.q * 64 + lane;
   const uint32_t deg = a.ndw[pos].x & NC_DEG_MASK;
   const int32_t lf = a.last_fire[pos];
-  const int32_t smod = s_done % 255;
-  const uint32_t sbase = a.slice_off[q] + lane * 4;
-  for (uint32_t k = 0; k < deg; ++k) {
-    const uint32_t j = sbase + (k >> 2) * 256 + (k & 3);
-    const uint32_t code = a.stamp[j];
-    if (code == 0u || nc_stamp_offset(smod, code) != 0) continue;
-    const nc_slot sl = a.slots[j];
-    if (sl.last_arr == s_done) continue;
-    const float tr_t = nc_lookup(a.t_post, a.len_post, s_done - lf);
-    a.slots[j].w = nc_stdp(sl.w, sl.flags, 0.0f, tr_t, pre_f, post_f, lr);
-    a.slots[j].last_arr = s_done;
+  const uint32_t nmw = (deg + 31) >> 5;
+  for (uint32_t w = 0; w < nmw; ++w) {
+    uint32_t bits = a.amask[rec.mask_off + w * 64 + lane];
+    while (bits) {
+      const uint32_t k = 32 * w + (uint32_t)__builtin_ctz(bits);
+      bits &= bits - 1;
+      const uint32_t j = nc_slot_of(rec.slot_off, lane, k);
+      const nc_slot r = a.slots[j];
+      if ((r.arr_inh >> 1) != s_done || r.last_arr == s_done) continue;
+      const float tr_t = nc_lookup(a.t_post, a.len_post, s_done - lf);
+      a.slots[j].w = nc_stdp(r.w, (uint32_t)r.arr_inh & 1u, 0.0f, tr_t, pre_f, post_f, lr);
+      a.slots[j].last_arr = s_done;
+    }
   }
 }
