@@ -242,16 +242,14 @@ struct nc_sim {
   int32_t* last_fire = nullptr;
   uint32_t *firings = nullptr, *nid = nullptr;
   uint2* ndw = nullptr;
-  uint32_t *out_beg = nullptr, *out_cnt = nullptr, *out_slot = nullptr;
-  uint8_t* out_delay = nullptr;
-  int32_t* out_arr = nullptr;
-  uint32_t* out_tq = nullptr;
+  uint2* out_row = nullptr;
+  nc_edge* edges = nullptr;
   uint2* pend = nullptr;
   uint32_t* pend_cnt = nullptr;
   uint32_t *slice_off = nullptr, *slice_net = nullptr;
   nc_slice_rec* active = nullptr;
   uint32_t *forced_in = nullptr, *forced_bg = nullptr;
-  unsigned long long* fmask = nullptr;
+  uint4* fired = nullptr;
 
   uint64_t ctr_rows = 0;
   uint32_t *raster = nullptr, *raster_cnt = nullptr;
@@ -340,15 +338,11 @@ nc_step_args base_args(nc_sim* sim) {
   a.e = sim->T.e;
   a.d_ref = sim->T.d_ref;
   a.r_rel = sim->T.r_rel;
-  a.out_beg = sim->out_beg;
-  a.out_cnt = sim->out_cnt;
-  a.out_slot = sim->out_slot;
-  a.out_delay = sim->out_delay;
-  a.out_arr = sim->out_arr;
-  a.out_tq = sim->out_tq;
+  a.out_row = sim->out_row;
+  a.edges = sim->edges;
   a.pend = sim->pend;
   a.pend_cnt = sim->pend_cnt;
-  a.fmask = sim->fmask;
+  a.fired = sim->fired;
   return a;
 }
 
@@ -457,12 +451,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMalloc(&sim->firings, P * 4), "hipMalloc firings");
     H(hipMalloc(&sim->ndw, P * sizeof(uint2)), "hipMalloc ndw");
     H(hipMalloc(&sim->nid, P * 4), "hipMalloc nid");
-    H(hipMalloc(&sim->out_beg, P * 4), "hipMalloc out_beg");
-    H(hipMalloc(&sim->out_cnt, P * 4), "hipMalloc out_cnt");
-    H(hipMalloc(&sim->out_slot, sim->max_out * 4), "hipMalloc out_slot");
-    H(hipMalloc(&sim->out_delay, sim->max_out), "hipMalloc out_delay");
-    H(hipMalloc(&sim->out_arr, sim->max_out * 4), "hipMalloc out_arr");
-    H(hipMalloc(&sim->out_tq, sim->max_out * 4), "hipMalloc out_tq");
+    H(hipMalloc(&sim->out_row, P * sizeof(uint2)), "hipMalloc out_row");
+    H(hipMalloc(&sim->edges, sim->max_out * sizeof(nc_edge)), "hipMalloc edges");
     H(hipMalloc(&sim->pend, sim->max_slots * sizeof(uint2)), "hipMalloc pend");
     H(hipMalloc(&sim->pend_cnt, Q * 4), "hipMalloc pend_cnt");
     H(hipMalloc(&sim->slice_off, Q * 4), "hipMalloc slice_off");
@@ -470,7 +460,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
     H(hipMalloc(&sim->forced_in, (P / 32) * 4), "hipMalloc forced_in");
     H(hipMalloc(&sim->forced_bg, (P / 32) * 4), "hipMalloc forced_bg");
-    H(hipMalloc(&sim->fmask, Q * 8), "hipMalloc fmask");
+    H(hipMalloc(&sim->fired, Q * sizeof(uint4)), "hipMalloc fired");
 
     H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
     if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
@@ -483,7 +473,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMemsetAsync(sim->ndw, 0, P * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
-    H(hipMemsetAsync(sim->fmask, 0, Q * 8, sim->stream), "memset");
+    H(hipMemsetAsync(sim->fired, 0, Q * sizeof(uint4), sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
     H(hipMemsetAsync(sim->ctr, 0, sim->ctr_rows * NC_NCTR * 8, sim->stream), "memset");
@@ -511,9 +501,8 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   (void)hipSetDevice(sim->device);
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   void* ptrs[] = {sim->amask, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
-                  sim->nid, sim->out_beg, sim->out_cnt, sim->out_slot,
-                  sim->out_delay, sim->out_arr, sim->out_tq, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
-                  sim->forced_bg, sim->fmask, sim->raster, sim->raster_cnt,
+                  sim->nid, sim->out_row, sim->edges, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->forced_bg, sim->fired, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p};
   for (void* p : ptrs)
@@ -656,17 +645,15 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       o += outcnt[pl];
     }
   }
-  std::vector<uint32_t> oslot(S), otq(S);
-  std::vector<uint8_t> odly(S);
+  std::vector<nc_edge> oedge(S);
   {
     std::vector<uint32_t> fillp(outbeg);
     for (uint32_t tg = 0; tg < net->n; ++tg) {
       for (int64_t k = net->in_ptr[tg]; k < net->in_ptr[tg + 1]; ++k) {
         uint32_t sp = L.pos_of_id[net->in_src[k]];
         uint32_t e = fillp[sp]++ - (uint32_t)o0;
-        oslot[e] = r.slot_of_syn[k];
-        odly[e] = net->delay[k];
-        otq[e] = (uint32_t)q0 + L.pos_of_id[tg] / 64;
+        const uint32_t tq = (uint32_t)q0 + L.pos_of_id[tg] / 64;
+        oedge[e] = nc_edge{r.slot_of_syn[k], (tq << 8) | net->delay[k], NC_NEVER, slice_off[tq - q0]};
       }
     }
   }
@@ -748,16 +735,10 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   for (uint32_t pl = 0; pl < npos; ++pl) ndwv[pl] = make_uint2(degw[pl], ingrp[pl]);
   NC_HIP(hipMemcpyAsync(sim->ndw + pos0, ndwv.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->nid + pos0, nidv.data(), npos * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->out_beg + pos0, outbeg.data(), npos * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->out_cnt + pos0, outcnt.data(), npos * 4, hipMemcpyHostToDevice, st));
-  if (S) {
-    NC_HIP(hipMemcpyAsync(sim->out_slot + o0, oslot.data(), S * 4, hipMemcpyHostToDevice, st));
-    NC_HIP(hipMemcpyAsync(sim->out_delay + o0, odly.data(), S, hipMemcpyHostToDevice, st));
-    std::vector<int32_t> never(S, NC_NEVER);
-    NC_HIP(hipMemcpyAsync(sim->out_arr + o0, never.data(), S * 4, hipMemcpyHostToDevice, st));
-    NC_HIP(hipStreamSynchronize(st));
-    NC_HIP(hipMemcpyAsync(sim->out_tq + o0, otq.data(), S * 4, hipMemcpyHostToDevice, st));
-  }
+  std::vector<uint2> orow(npos);
+  for (uint32_t pl = 0; pl < npos; ++pl) orow[pl] = make_uint2(outbeg[pl], outcnt[pl]);
+  NC_HIP(hipMemcpyAsync(sim->out_row + pos0, orow.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
+  if (S) NC_HIP(hipMemcpyAsync(sim->edges + o0, oedge.data(), S * sizeof(nc_edge), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, L.nq * 4, st));
   NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), L.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), L.nq * 4, hipMemcpyHostToDevice, st));

@@ -49,6 +49,15 @@ struct nc_slice_rec {
   uint32_t mask_off;  // first active-mask word of the slice
 };
 
+// One outgoing synapse, in its source's out-list.
+struct nc_edge {
+  uint32_t j;      // slot of the synapse at its target
+  uint32_t tq_d;   // target slice << 8 | delay in steps
+  int32_t arr;     // arrival step of the last accepted spike (NC_NEVER: none);
+                   // the latch holds until arrival + release window
+  uint32_t tbase;  // first pending-list entry of the target slice
+};
+
 struct nc_group {
   uint32_t tbl_off;  // first threshold entry
   uint32_t period;   // entries
@@ -76,7 +85,7 @@ struct nc_step_args {
   uint32_t* forced_in;
   uint32_t* forced_bg;
   int32_t use_forced;
-  unsigned long long* fmask;  // fired lanes of each active slice (by active-list index), this step
+  uint4* fired;  // per active slice (by active-list index): fired-lane mask lo/hi, step, slice
   uint32_t* raster;           // (pos, step) pairs
   uint32_t* raster_cnt;
   uint32_t raster_cap;
@@ -91,13 +100,8 @@ struct nc_step_args {
   double c0, e;
   int32_t d_ref, r_rel;
   int32_t chunk_i;
-  const uint32_t* out_beg;
-  const uint32_t* out_cnt;
-  const uint32_t* out_slot;
-  const uint8_t* out_delay;
-  int32_t* out_arr;        // source order: arrival step of the synapse's last accepted spike
-                           // (NC_NEVER: none); its latch holds until arrival + release window
-  const uint32_t* out_tq;  // target slice of each out-synapse
+  const uint2* out_row;     // per position: first out-edge, out-degree
+  nc_edge* edges;           // out-edges, rows in ascending target id
   uint2* pend;             // per slice, at its slot range: (slot, arrival step) of spikes in flight
   uint32_t* pend_cnt;      // per slice: entries in its pending list
   const uint32_t* ev_pos;  // host events of this step
@@ -330,7 +334,8 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   }
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
-  if (wave_in && lane == 0) a.fmask[wave] = live ? fm : 0ull;
+  if (wave_in && lane == 0)
+    a.fired[wave] = live ? make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q) : make_uint4(0u, 0u, 0u, 0u);
   // The passes below touch slot records and mask words this wave wrote above,
   // from other lanes: make those writes visible first.
   if (fm || pend_n) {
@@ -411,36 +416,39 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
 // writer per synapse (its source), latch permitting.  Accepted spikes latch
 // depolarisation and arrival step into the target slot and join the target
 // slice's pending list.
+__device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, const nc_edge& ed, int32_t s,
+                                             uint32_t& n_acc) {
+  // a spike still in flight or in its release window (:728)
+  if (ed.arr + a.r_rel > s) return;
+  const int32_t arr = s + (int32_t)(ed.tq_d & 0xffu);
+  const nc_slot r = a.slots[ed.j];
+  a.slots[ed.j].depol = a.depol_k * r.w;
+  a.slots[ed.j].arr_inh = arr * 2 + (r.arr_inh & 1);
+  a.edges[e].arr = arr;
+  const uint32_t at = atomicAdd(&a.pend_cnt[ed.tq_d >> 8], 1u);
+  a.pend[ed.tbase + at] = make_uint2(ed.j, (uint32_t)arr);
+  ++n_acc;
+}
+
 __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
   uint32_t n_out = 0, n_acc = 0;
-  unsigned long long m = wave < a.n_active_slices ? a.fmask[wave] : 0ull;
-  if (m) {
-    const nc_slice_rec rec = a.active_slices[wave];
-    const nc_netdesc* nd = &a.nets[rec.net];
-    const int32_t s = nd->base_step + a.chunk_i + 1;
-    while (m) {
-      const uint32_t l = (uint32_t)__builtin_ctzll(m);
-      m &= m - 1;
-      const uint32_t pos = rec.q * 64 + l;
-      const uint32_t beg = a.out_beg[pos], cnt = a.out_cnt[pos];
-      n_out += (lane < cnt) ? ((cnt - lane + 63) >> 6) : 0u;
-      for (uint32_t k = lane; k < cnt; k += 64) {
-        const uint32_t e = beg + k;
-        // a spike still in flight or in its release window (:728)
-        if (a.out_arr[e] + a.r_rel > s) continue;
-        const uint32_t j = a.out_slot[e];
-        const int32_t arr = s + (int32_t)a.out_delay[e];
-        const uint32_t tq = a.out_tq[e];
-        const nc_slot r = a.slots[j];
-        a.slots[j].depol = a.depol_k * r.w;
-        a.slots[j].arr_inh = arr * 2 + (r.arr_inh & 1);
-        a.out_arr[e] = arr;
-        const uint32_t at = atomicAdd(&a.pend_cnt[tq], 1u);
-        a.pend[a.slice_off[tq] + at] = make_uint2(j, (uint32_t)arr);
-        ++n_acc;
-      }
+  const uint4 f = wave < a.n_active_slices ? a.fired[wave] : make_uint4(0u, 0u, 0u, 0u);
+  unsigned long long m = ((unsigned long long)f.y << 32) | f.x;
+  const int32_t s = (int32_t)f.z;
+  while (m) {
+    const uint32_t l = (uint32_t)__builtin_ctzll(m);
+    m &= m - 1;
+    const uint2 row = a.out_row[f.w * 64 + l];
+    n_out += (lane < row.y) ? ((row.y - lane + 63) >> 6) : 0u;
+    for (uint32_t k = lane; k < row.y; k += 128) {
+      const bool two = k + 64 < row.y;
+      const nc_edge e0 = a.edges[row.x + k];
+      nc_edge e1 = e0;
+      if (two) e1 = a.edges[row.x + k + 64];
+      nc_fire_edge(a, row.x + k, e0, s, n_acc);
+      if (two) nc_fire_edge(a, row.x + k + 64, e1, s, n_acc);
     }
   }
   if (a.ctr) {
