@@ -27,22 +27,43 @@ METRIC = "synapse updates/s (% HBM roofline); served sims/s + p50/p99 vs plain"
 HBM_PEAK_GBS = 8000.0
 
 
-def neuron_kernel_bytes(n_neurons, n_syn, c0, c1, steps):
-    """Algorithmic bytes per launch of the neuron-step kernel, from the
-    device counters over the timed steps:
-      16 B per neuron  (in-degree word 4, V read+write 8, lastFire read 4)
-       1 B per synapse (spike-latch byte scanned every step)
-       4 B per active slot-step (latched depolarisation read)
-      13 B per processed arrival (w read+write 8, last-arrival write 4,
-                                  latch clear 1)
-      12 B per incoming synapse of a fired neuron (post-side STDP:
-                                  last-arrival read 4, w read+write 8)
-      12 B per fired neuron (lastFire write 4, firings read+write 8)
-       8 B per 64 neurons (fired bitmask of the slice)"""
-    d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree")}
-    A, R, F, Fin = d["active_slot_steps"], d["arrivals"], d["fired"], d["fired_in_degree"]
-    b = 16.0 * n_neurons + 1.0 * n_syn + 4.0 * A + 13.0 * R + 12.0 * Fin + 12.0 * F + 8.0 * n_neurons / 64
-    return b, {"A": A, "R": R, "F": F, "F_in": Fin}
+def neuron_kernel_bytes(n_neurons, c0, c1, steps):
+    """Algorithmic bytes per launch of the neuron-step kernel (k_neuron), from
+    the engine's event counters over the timed steps:
+      20 B per neuron        in-degree/flags + input group 8, V read+write 8,
+                             lastFire read 4
+      36 B per 64 neurons    active-slice record 16, pending count 4,
+                             fired-lane record write 16
+       4 B per mask word     active-slot bit words read (ceil(in-degree/32)
+                             per neuron) and 4 B per word written back
+      16 B per active slot   slot record {w, depol, lastArrival, arrival}
+      16 B per arrival       arrival STDP (w, lastArrival) 8, arrival bit 8
+      20 B per in-synapse    post-side STDP of a fired neuron: record read 16,
+           of a fired neuron w write 4
+      12 B per fired neuron  firings read+write 8, lastFire write 4
+       8 B per pending entry scanned (spikes in flight to the slice)"""
+    d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree",
+                                               "pending_scanned", "mask_words_read", "mask_words_written")}
+    b = (20.0 * n_neurons + 36.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
+         + 16.0 * d["active_slot_steps"] + 16.0 * d["arrivals"] + 20.0 * d["fired_in_degree"] + 12.0 * d["fired"]
+         + 8.0 * d["pending_scanned"])
+    return b, {"A": d["active_slot_steps"], "R": d["arrivals"], "F": d["fired"], "F_in": d["fired_in_degree"],
+               "pending": d["pending_scanned"], "mask_words": d["mask_words_read"]}
+
+
+def copy_calibration(mib):
+    """A plain streaming kernel of known size (torch.neg over `mib` MiB of
+    float32 into a second buffer: reads and writes exactly that many bytes),
+    for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
+                      "launches": 5}), flush=True)
 
 
 def drive_fires(drive, ids, groups, n_corr_groups, steps, keep):
@@ -131,7 +152,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="leave HIP events out of the timed region (for external profilers)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: every box in this setup has a single MI355X; "
               "independent networks shard across GPUs without communication, see DESIGN.md",
@@ -174,7 +200,7 @@ def main():
     print("[bench] timed %d steps in %.3fs" % (args.steps, elapsed), file=sys.stderr, flush=True)
 
     value = S * args.steps / elapsed
-    bytes_launch, ev = neuron_kernel_bytes(N, S, c0, c1, args.steps)
+    bytes_launch, ev = neuron_kernel_bytes(N, c0, c1, args.steps)
     avg_launch_s = (k_ms / 1000.0) / launches if launches else None
     achieved = bytes_launch / avg_launch_s / 1e9 if avg_launch_s else None
     traffic = None
@@ -204,7 +230,8 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload, "neurons": N, "synapses": S, "dt_ms": params.dt,
                    "stdp": True, "seed": 0, "fired_per_step": ev["F"], "rate_per_step": rate,
-                   "arrivals_per_step": ev["R"], "active_slots_per_step": ev["A"]},
+                   "arrivals_per_step": ev["R"], "active_slots_per_step": ev["A"],
+                   "pending_per_step": ev["pending"]},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
                      "kernel": "k_neuron", "algorithmic_bytes_per_launch": bytes_launch,

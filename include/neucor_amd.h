@@ -231,8 +231,12 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
 /* Event counters since creation, summed over all networks:
  * [0] steps*neurons, [1] synapse-steps (sum of S per step),
  * [2] active slot-steps, [3] arrivals, [4] fired neurons,
- * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes. */
-nc_status nc_read_counters(nc_sim* sim, uint64_t out[8]);
+ * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes,
+ * [8] pending-list entries scanned, [9] active-mask words read,
+ * [10] active-mask words written back.
+ * Writes the first n (at most NC_NUM_COUNTERS) into out. */
+#define NC_NUM_COUNTERS 11
+nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
 
 /* Block until every step enqueued on the nc_sim's stream has finished. */
 nc_status nc_sim_synchronize(nc_sim* sim);

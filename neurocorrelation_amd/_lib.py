@@ -11,6 +11,7 @@ LIB_PATH = os.path.join(HERE, "libneucor_amd.so")
 
 NC_OK = 0
 NC_NEVER_STEP = -(1 << 30)
+NC_NUM_COUNTERS = 11
 STATUS = {
     0: "NC_OK", 1: "NC_ERR_ARG", 2: "NC_ERR_RANGE", 3: "NC_ERR_HIP", 4: "NC_ERR_OOM",
     5: "NC_ERR_CAPACITY", 6: "NC_ERR_STATE", 7: "NC_ERR_NO_DEVICE",
@@ -86,7 +87,7 @@ SIGNATURES = {
     "nc_read_synapse_state": (C.c_int, [P, C.c_int32, i32p, i32p, f32p, C.c_uint64]),
     "nc_read_raster": (C.c_int, [P, C.c_int32, u32p, u32p, C.c_uint64, u64p]),
     "nc_reset_activities": (C.c_int, [P, C.c_int32]),
-    "nc_read_counters": (C.c_int, [P, u64p]),
+    "nc_read_counters": (C.c_int, [P, u64p, C.c_uint32]),
     "nc_sim_synchronize": (C.c_int, [P]),
     "nc_set_kernel_timing": (C.c_int, [P, C.c_int]),
     "nc_read_kernel_timing": (C.c_int, [P, C.c_uint32, u64p, C.POINTER(C.c_double)]),

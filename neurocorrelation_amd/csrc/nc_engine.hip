@@ -17,6 +17,7 @@
 //   out-CSR: per position a row of slot indices, u8 delays, target slices
 //            and the source-side copy of each synapse's latch, for the
 //            fan-out.
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <string.h>
@@ -200,6 +201,7 @@ struct NetRec {
   uint32_t q0 = 0, nq = 0;
   uint64_t slot0 = 0, nslots = 0, out0 = 0, mask0 = 0, nmwords = 0;
   uint32_t act0 = 0;  // first record of this network in the active-slice list
+  uint64_t mwords_step = 0;  // active-mask words k_neuron reads per step
   std::vector<uint32_t> pos_of_id;    // global positions
   std::vector<uint32_t> slot_of_syn;  // global slot per synapse, export order
   std::vector<uint32_t> id_of_pos;    // local, nq*64
@@ -269,7 +271,7 @@ struct nc_sim {
   bool active_dirty = true, groups_dirty = true;
   std::vector<uint32_t> slice_net_host;
   uint64_t raster_read = 0;
-  uint64_t host_neuron_steps = 0, host_synapse_steps = 0;
+  uint64_t host_neuron_steps = 0, host_synapse_steps = 0, host_mword_steps = 0;
   // Host staging of the per-call uploads; reused only after upload_done.
   std::vector<nc_netdesc> h_nds;
   std::vector<uint32_t> h_ev_pos;
@@ -659,10 +661,13 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   }
   // Drive.
   std::vector<uint32_t> ingrp(npos, 0), degw(npos, 0), nidv(npos, ~0u);
+  r.mwords_step = 0;
   for (uint32_t pl = 0; pl < npos; ++pl) {
     uint32_t idl = L.id_of_pos[pl];
     if (idl == ~0u) continue;
-    degw[pl] = (uint32_t)(net->in_ptr[idl + 1] - net->in_ptr[idl]) | NC_DEG_VALID;
+    const uint32_t dg = (uint32_t)(net->in_ptr[idl + 1] - net->in_ptr[idl]);
+    degw[pl] = dg | NC_DEG_VALID;
+    r.mwords_step += (dg + 31) / 32;
     nidv[pl] = idl;
   }
   nc_netdesc nd{};
@@ -911,15 +916,17 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
       hipEvent_t e0 = nullptr, e1 = nullptr;
       const bool timed = (sim->kernel_timing >> kind) & 1;
       if (timed) {
+        // start/stop events carried by the dispatch itself: they time the
+        // kernel, not the queue around it
         e0 = get_event(sim, nev_used++);
         e1 = get_event(sim, nev_used++);
         if (!e0 || !e1) return fail(NC_ERR_HIP, "hipEventCreate failed");
-        NC_HIP(hipEventRecord(e0, st));
+        hipExtLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, e0, e1, 0, a);
+      } else {
+        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, a);
       }
-      hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, a);
       NC_HIP(hipGetLastError());
       if (timed) {
-        NC_HIP(hipEventRecord(e1, st));
         sim->k_launch[kind]++;
         timed_kinds.push_back(kind);
       }
@@ -946,6 +953,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     uint32_t run = (uint32_t)nds[i].remaining;
     sim->host_neuron_steps += (uint64_t)run * r.n;
     sim->host_synapse_steps += (uint64_t)run * r.S;
+    sim->host_mword_steps += (uint64_t)run * r.mwords_step;
     r.steps_done += run;
     if (r.steps_left != UINT32_MAX) r.steps_left -= run;
   }
@@ -1158,19 +1166,23 @@ extern "C" nc_status nc_reset_activities(nc_sim* sim, int32_t net_id) {
   return NC_OK;
 }
 
-extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t out[8]) {
-  if (!sim || !out) return fail(NC_ERR_ARG, "null argument");
+extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) {
+  if (!sim || (!out && n)) return fail(NC_ERR_ARG, "null argument");
+  if (n > NC_NUM_COUNTERS) return fail(NC_ERR_RANGE, "at most NC_NUM_COUNTERS counters");
   NC_HIP(hipSetDevice(sim->device));
   std::vector<unsigned long long> h(sim->ctr_rows * NC_NCTR);
   NC_HIP(hipMemcpyAsync(h.data(), sim->ctr, h.size() * 8, hipMemcpyDeviceToHost, sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
-  for (int k = 0; k < NC_NCTR; ++k) {
+  uint64_t all[NC_NUM_COUNTERS] = {};
+  for (int k = 0; k < NC_NCTR && k < NC_NUM_COUNTERS; ++k) {
     uint64_t v = 0;
     for (uint64_t r = 0; r < sim->ctr_rows; ++r) v += h[r * NC_NCTR + k];
-    out[k] = v;
+    all[k] = v;
   }
-  out[0] = sim->host_neuron_steps;
-  out[1] = sim->host_synapse_steps;
+  all[0] = sim->host_neuron_steps;
+  all[1] = sim->host_synapse_steps;
+  all[9] = sim->host_mword_steps;
+  for (uint32_t k = 0; k < n; ++k) out[k] = all[k];
   return NC_OK;
 }
 

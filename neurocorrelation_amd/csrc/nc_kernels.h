@@ -28,7 +28,7 @@
 #define NC_LDS_TAB 2048  // trace-table entries staged in LDS per block
 #define NC_WAP_K 8       // AP-table entries passed as kernel arguments
 #define NC_MW 4          // active-mask words (32 slots each) a lane holds per pass
-#define NC_NCTR 8
+#define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
 
 // Per-network descriptor, rewritten by the host before each batch of steps.
 struct nc_netdesc {
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   __syncthreads();
   const float* tpre = a.tabs_in_lds ? sh_pre : a.t_pre;
   const float* tpost = a.tabs_in_lds ? sh_post : a.t_post;
-  uint32_t n_active = 0, n_arr = 0;
+  uint32_t n_active = 0, n_arr = 0, n_mwb = 0;
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
   // that arrived at s-R .. s-1) in ascending source order.  Each slot record
@@ -281,7 +281,10 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
     }
 #pragma unroll
     for (int u = 0; u < NC_MW; ++u)
-      if (m[u] != m0[u]) mrow[(w0 + u) * 64] = m[u];
+      if (m[u] != m0[u]) {
+        mrow[(w0 + u) * 64] = m[u];
+        ++n_mwb;
+      }
   }
 
   // charge_passive (src/NeuCor.cpp:677-680).
@@ -389,7 +392,8 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
       }
       const bool keep = t < pend_n && !arrive;
       const unsigned long long km = __ballot(keep);
-      if (keep) pl[kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull))] = e;
+      const uint32_t to = kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull));
+      if (keep && to != t) pl[to] = e;
       kept += (uint32_t)__builtin_popcountll(km);
     }
     if (lane == 0) a.pend_cnt[q] = kept;
@@ -408,6 +412,8 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
     if (lane == 0 && fm)
       atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
     nc_count32(a.ctr, 5, n_post);
+    nc_count32(a.ctr, 10, n_mwb);
+    if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 8], (unsigned long long)pend_n);
   }
 }
 

@@ -188,10 +188,11 @@ class Simulation:
         check(lib().nc_reset_activities(self._h, net_id))
 
     def counters(self):
-        out = np.zeros(8, np.uint64)
-        check(lib().nc_read_counters(self._h, ptr(out, C.c_uint64)))
+        out = np.zeros(_lib.NC_NUM_COUNTERS, np.uint64)
+        check(lib().nc_read_counters(self._h, ptr(out, C.c_uint64), len(out)))
         keys = ["neuron_steps", "synapse_steps", "active_slot_steps", "arrivals", "fired",
-                "fired_in_degree", "fired_out_degree", "accepted_spikes"]
+                "fired_in_degree", "fired_out_degree", "accepted_spikes", "pending_scanned",
+                "mask_words_read", "mask_words_written"]
         return dict(zip(keys, (int(x) for x in out)))
 
     def synchronize(self):
