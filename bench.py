@@ -25,6 +25,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "synapse updates/s (% HBM roofline); served sims/s + p50/p99 vs plain"
 HBM_PEAK_GBS = 8000.0
+TIMING_EVERY = 10
 
 
 def neuron_kernel_bytes(n_neurons, c0, c1, steps):
@@ -152,6 +153,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true",
                     help="leave HIP events out of the timed region (for external profilers)")
+    ap.add_argument("--no-counts", action="store_true",
+                    help="skip the counting replay (profiler runs: every launch is then uninstrumented)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -177,37 +180,63 @@ def main():
     print("[bench] %s: %d neurons, %d synapses, setup %.1fs" % (args.workload, N, S, setup_s),
           file=sys.stderr, flush=True)
 
-    done = 0
-    while done < args.warmup:
-        k = min(args.chunk, args.warmup - done)
-        sim.step(k)
-        done += k
+    def run(sim_, n):
+        done_ = 0
+        while done_ < n:
+            k_ = min(args.chunk, n - done_)
+            sim_.step(k_)
+            done_ += k_
+
+    run(sim, args.warmup)
     sim.synchronize()
-    c0 = sim.counters()
     if not args.no_kernel_timing:
-        sim.set_kernel_timing(1 << 1)  # HIP events around every neuron-step launch
+        # start/stop events carried by the neuron-step dispatch of every
+        # TIMING_EVERY-th step (events on every launch would slow the queue)
+        sim.set_kernel_timing(1 << 1, TIMING_EVERY)
     t_start = time.perf_counter()
-    done = 0
-    while done < args.steps:
-        k = min(args.chunk, args.steps - done)
-        sim.step(k)
-        done += k
+    run(sim, args.steps)
     sim.synchronize()
     elapsed = time.perf_counter() - t_start
-    c1 = sim.counters()
     launches, k_ms = sim.kernel_timing(1)
     sim.set_kernel_timing(0)
     print("[bench] timed %d steps in %.3fs" % (args.steps, elapsed), file=sys.stderr, flush=True)
+    if not args.no_kernel_timing:
+        t1 = time.perf_counter()
+        run(sim, args.steps)
+        sim.synchronize()
+        print("[bench] next %d steps without kernel events: %.2f us/step (timed run: %.2f)" %
+              (args.steps, (time.perf_counter() - t1) * 1e6 / args.steps, elapsed * 1e6 / args.steps),
+              file=sys.stderr, flush=True)
+    sim.close()
+
+    # Event counts of the timed steps: the same network, drive and steps again
+    # with the device counters on (the step is deterministic; counting is kept
+    # out of the timed run).
+    if args.no_counts:
+        c0 = c1 = None
+    else:
+        sim = nc.Simulation.for_network(net)
+        sim.add(net, params, drive)
+        run(sim, args.warmup)
+        sim.set_counting(True)
+        c0 = sim.counters()
+        run(sim, args.steps)
+        sim.synchronize()
+        c1 = sim.counters()
+        sim.close()
 
     value = S * args.steps / elapsed
-    bytes_launch, ev = neuron_kernel_bytes(N, c0, c1, args.steps)
+    if c0 is not None:
+        bytes_launch, ev = neuron_kernel_bytes(N, c0, c1, args.steps)
+    else:
+        bytes_launch, ev = None, {k: None for k in ("A", "R", "F", "F_in", "pending", "mask_words")}
     avg_launch_s = (k_ms / 1000.0) / launches if launches else None
-    achieved = bytes_launch / avg_launch_s / 1e9 if avg_launch_s else None
+    achieved = bytes_launch / avg_launch_s / 1e9 if (avg_launch_s and bytes_launch) else None
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
         traffic = json.load(open(tpath)).get("neuron_step_bytes_per_launch")
-    rate = ev["F"] / N
+    rate = ev["F"] / N if ev["F"] is not None else None
 
     cpu = None
     if not args.no_cpu_baseline:

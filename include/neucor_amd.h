@@ -234,18 +234,25 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes,
  * [8] pending-list entries scanned, [9] active-mask words read,
  * [10] active-mask words written back.
+ * [0], [1] and [9] are kept by the host; the others are device counters that
+ * advance only while counting is on (nc_set_counting).
  * Writes the first n (at most NC_NUM_COUNTERS) into out. */
 #define NC_NUM_COUNTERS 11
 nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
 
+/* Device event counters on (enable != 0) or off (the default: the step
+ * kernels then carry no instrumentation). */
+nc_status nc_set_counting(nc_sim* sim, int enable);
+
 /* Block until every step enqueued on the nc_sim's stream has finished. */
 nc_status nc_sim_synchronize(nc_sim* sim);
 
-/* Per-kernel timing (HIP events around each launch, on the engine's stream)
- * for the kernels in kernel_mask (bit k: 0 drive, 1 neuron step, 2 post
- * STDP, 3 fan-out; 0 = off); nc_read_kernel_timing returns launches and
- * total milliseconds since the last nc_set_kernel_timing call. */
-nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask);
+/* Per-kernel timing for the kernels in kernel_mask (bit k: 0 drive, 1 neuron
+ * step, 3 fan-out; 0 = off): start/stop events carried by the dispatch of
+ * every `every`-th step (1 = all steps; the events cost the queue a little
+ * time per timed launch).  nc_read_kernel_timing returns the timed launches
+ * and their total milliseconds since the last nc_set_kernel_timing call. */
+nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask, uint32_t every);
 nc_status nc_read_kernel_timing(nc_sim* sim, uint32_t kernel, uint64_t* launches,
                                 double* total_ms);
 

@@ -88,8 +88,9 @@ SIGNATURES = {
     "nc_read_raster": (C.c_int, [P, C.c_int32, u32p, u32p, C.c_uint64, u64p]),
     "nc_reset_activities": (C.c_int, [P, C.c_int32]),
     "nc_read_counters": (C.c_int, [P, u64p, C.c_uint32]),
+    "nc_set_counting": (C.c_int, [P, C.c_int]),
     "nc_sim_synchronize": (C.c_int, [P]),
-    "nc_set_kernel_timing": (C.c_int, [P, C.c_int]),
+    "nc_set_kernel_timing": (C.c_int, [P, C.c_int, C.c_uint32]),
     "nc_read_kernel_timing": (C.c_int, [P, C.c_uint32, u64p, C.POINTER(C.c_double)]),
     "nc_last_error": (C.c_char_p, []),
 }

@@ -233,6 +233,8 @@ struct nc_sim {
   Tables T;
   uint64_t gstep = 0;
   int kernel_timing = 0;  // bit mask of timed kernels
+  uint32_t timing_every = 1;
+  uint64_t timing_step = 0;
   uint64_t k_launch[4] = {0, 0, 0, 0};
   double k_ms[4] = {0, 0, 0, 0};
   std::vector<hipEvent_t> ev_pool;
@@ -240,6 +242,7 @@ struct nc_sim {
   // device buffers
   nc_slot* slots = nullptr;
   uint32_t* amask = nullptr;
+  uint32_t* asum = nullptr;
   float* V = nullptr;
   int32_t* last_fire = nullptr;
   uint32_t *firings = nullptr, *nid = nullptr;
@@ -256,6 +259,7 @@ struct nc_sim {
   uint64_t ctr_rows = 0;
   uint32_t *raster = nullptr, *raster_cnt = nullptr;
   unsigned long long* ctr = nullptr;
+  bool counting = false;
   nc_netdesc* d_nets = nullptr;
   float *t_pre = nullptr, *t_post = nullptr, *w_ap = nullptr;
   DevBuf<nc_group> d_groups;
@@ -271,6 +275,8 @@ struct nc_sim {
   bool active_dirty = true, groups_dirty = true;
   std::vector<uint32_t> slice_net_host;
   uint64_t raster_read = 0;
+  bool act_identity = false;  // active record i is slice act_q0 + i
+  uint32_t act_q0 = 0;
   uint64_t host_neuron_steps = 0, host_synapse_steps = 0, host_mword_steps = 0;
   // Host staging of the per-call uploads; reused only after upload_done.
   std::vector<nc_netdesc> h_nds;
@@ -308,6 +314,7 @@ nc_step_args base_args(nc_sim* sim) {
   nc_step_args a{};
   a.slots = sim->slots;
   a.amask = sim->amask;
+  a.asum = sim->asum;
   a.V = sim->V;
   a.last_fire = sim->last_fire;
   a.firings = sim->firings;
@@ -316,6 +323,8 @@ nc_step_args base_args(nc_sim* sim) {
   a.slice_off = sim->slice_off;
   a.active_slices = sim->active;
   a.n_active_slices = (uint32_t)sim->active_host.size();
+  a.act_identity = sim->act_identity ? 1u : 0u;
+  a.act_q0 = sim->act_q0;
   a.nets = sim->d_nets;
   a.groups = sim->d_groups.p;
   a.thr = sim->d_thr.p;
@@ -324,7 +333,7 @@ nc_step_args base_args(nc_sim* sim) {
   a.raster = sim->raster;
   a.raster_cnt = sim->raster_cnt;
   a.raster_cap = sim->raster_cap;
-  a.ctr = sim->ctr;
+  a.ctr = sim->counting ? sim->ctr : nullptr;
   a.t_pre = sim->t_pre;
   a.t_post = sim->t_post;
   a.w_ap = sim->w_ap;
@@ -365,6 +374,10 @@ nc_status rebuild_active(nc_sim* sim) {
     for (uint32_t q = r.q0; q < r.q0 + r.nq; ++q)
       sim->active_host.push_back(nc_slice_rec{q, sim->slice_off_host[q], (uint32_t)i, sim->slice_moff_host[q]});
   }
+  sim->act_identity = true;
+  sim->act_q0 = sim->active_host.empty() ? 0u : sim->active_host[0].q;
+  for (size_t i = 0; i < sim->active_host.size(); ++i)
+    if (sim->active_host[i].q != sim->act_q0 + (uint32_t)i) sim->act_identity = false;
   if (!sim->active_host.empty())
     NC_HIP(hipMemcpy(sim->active, sim->active_host.data(), sim->active_host.size() * sizeof(nc_slice_rec),
                      hipMemcpyHostToDevice));
@@ -447,6 +460,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
   if (st == NC_OK) {
     sim->max_mwords = sim->max_slots / 32 + 64 * Q;
     H(hipMalloc(&sim->amask, sim->max_mwords * 4), "hipMalloc amask");
+    H(hipMalloc(&sim->asum, P * 4), "hipMalloc asum");
     H(hipMalloc(&sim->slots, sim->max_slots * sizeof(nc_slot)), "hipMalloc slots");
     H(hipMalloc(&sim->V, P * 4), "hipMalloc V");
     H(hipMalloc(&sim->last_fire, P * 4), "hipMalloc lastFire");
@@ -472,6 +486,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
   }
   if (st == NC_OK) {
     H(hipMemsetAsync(sim->amask, 0, sim->max_mwords * 4, sim->stream), "memset");
+    H(hipMemsetAsync(sim->asum, 0, P * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->ndw, 0, P * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
@@ -502,7 +517,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (!sim) return;
   (void)hipSetDevice(sim->device);
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
-  void* ptrs[] = {sim->amask, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
+  void* ptrs[] = {sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
                   sim->nid, sim->out_row, sim->edges, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
@@ -733,6 +748,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   hipStream_t st = sim->stream;
   NC_HIP(hipMemcpyAsync(sim->slots + sl0, hs.data(), L.nslots * sizeof(nc_slot), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemsetAsync(sim->amask + mk0, 0, L.nmwords * 4, st));
+  NC_HIP(hipMemsetAsync(sim->asum + pos0, 0, npos * 4, st));
   NC_HIP(hipMemcpyAsync(sim->V + pos0, V0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->firings + pos0, zero.data(), npos * 4, hipMemcpyHostToDevice, st));
@@ -914,7 +930,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     auto launch = [&](int kind, auto fn, uint32_t grid) -> nc_status {
       if (grid == 0) return NC_OK;
       hipEvent_t e0 = nullptr, e1 = nullptr;
-      const bool timed = (sim->kernel_timing >> kind) & 1;
+      const bool timed = ((sim->kernel_timing >> kind) & 1) && (sim->timing_step % sim->timing_every) == 0;
       if (timed) {
         // start/stop events carried by the dispatch itself: they time the
         // kernel, not the queue around it
@@ -938,6 +954,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     if (s1 == NC_OK) s1 = launch(3, k_fanout, blocks1);
     if (s1 != NC_OK) return s1;
     sim->gstep++;
+    sim->timing_step++;
   }
   if (!timed_kinds.empty()) {
     NC_HIP(hipEventSynchronize(sim->ev_pool[nev_used - 1]));
@@ -1186,6 +1203,12 @@ extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) {
   return NC_OK;
 }
 
+extern "C" nc_status nc_set_counting(nc_sim* sim, int enable) {
+  if (!sim) return fail(NC_ERR_ARG, "null sim");
+  sim->counting = enable != 0;
+  return NC_OK;
+}
+
 extern "C" nc_status nc_sim_synchronize(nc_sim* sim) {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   NC_HIP(hipSetDevice(sim->device));
@@ -1193,9 +1216,11 @@ extern "C" nc_status nc_sim_synchronize(nc_sim* sim) {
   return NC_OK;
 }
 
-extern "C" nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask) {
+extern "C" nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask, uint32_t every) {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   sim->kernel_timing = kernel_mask & 0xF;
+  sim->timing_every = every ? every : 1;
+  sim->timing_step = 0;
   for (int k = 0; k < 4; ++k) {
     sim->k_launch[k] = 0;
     sim->k_ms[k] = 0.0;

@@ -68,6 +68,8 @@ struct nc_group {
 struct nc_step_args {
   nc_slot* slots;
   uint32_t* amask;  // per slice: [mask word][lane] bits of active slots
+  uint32_t* asum;   // per position: bit b set if mask words [b*G, b*G+G) may hold bits
+                    // (G = 1 up to 32 words, else ceil(words/32))
   float* V;
   int32_t* last_fire;
   uint32_t* firings;
@@ -76,6 +78,8 @@ struct nc_step_args {
   const uint32_t* slice_off;
   const nc_slice_rec* active_slices;
   uint32_t n_active_slices;
+  uint32_t act_identity;  // active record i is slice act_q0 + i (one contiguous run)
+  uint32_t act_q0;
   const nc_netdesc* nets;
   const nc_group* groups;
   const uint32_t* thr;
@@ -199,15 +203,28 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
   const uint32_t lane = threadIdx.x & 63;
   const bool wave_in = wave < a.n_active_slices;
-  const nc_slice_rec rec = a.active_slices[wave_in ? wave : 0];
-  const nc_netdesc nd = a.nets[rec.net];
-  const uint32_t q = rec.q;
+  const uint32_t widx = wave_in ? wave : 0u;
+  const nc_slice_rec rec = a.active_slices[widx];
+  // With one contiguous run of slices the per-neuron loads need not wait
+  // for the slice record.
+  const uint32_t q = a.act_identity ? a.act_q0 + widx : rec.q;
   const uint32_t pos = q * 64 + lane;
+  uint2 nw = a.ndw[pos];
+  float V = a.V[pos];
+  int32_t lf = a.last_fire[pos];
+  uint32_t pend_n = a.pend_cnt[q];
+  uint32_t sm = a.asum[pos];
+  const nc_netdesc nd = a.nets[rec.net];
   const bool live = wave_in && a.chunk_i < nd.remaining;
-  const uint2 nw = live ? a.ndw[pos] : make_uint2(0u, 0u);
-  float V = live ? a.V[pos] : 0.0f;
-  int32_t lf = live ? a.last_fire[pos] : NC_NEVER;
-  const uint32_t pend_n = live ? a.pend_cnt[q] : 0u;
+  if (!live) {
+    nw = make_uint2(0u, 0u);
+    pend_n = 0u;
+    sm = 0u;
+  }
+  // First 64 pending entries, fetched now and used at the end of the step.
+  uint2* const pl = a.pend + rec.slot_off;
+  uint2 pe0 = make_uint2(0u, 0u);
+  if (lane < pend_n) pe0 = pl[lane];
   const int32_t s = nd.base_step + a.chunk_i + 1;
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
@@ -227,22 +244,25 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   // carries the latched depolarisation and the arrival step, so one fetch
   // serves the contribution, the arrival STDP and the latch release.
   const uint32_t nmw = (deg + 31) >> 5;
-  uint32_t* mrow = a.amask + rec.mask_off + lane;
-  for (uint32_t w0 = 0; w0 < nmw; w0 += NC_MW) {
-    uint32_t m[NC_MW], m0[NC_MW];
-#pragma unroll
-    for (int u = 0; u < NC_MW; ++u) m[u] = m0[u] = (w0 + u < nmw) ? mrow[(w0 + u) * 64] : 0u;
-#pragma unroll
-    for (int u = 0; u < NC_MW; ++u) {
-      uint32_t bits = m[u];
+  const uint32_t G = nmw <= 32 ? 1u : (nmw + 31) >> 5;  // mask words per summary bit
+  uint32_t* const mrow = a.amask + rec.mask_off + lane;
+  uint32_t sm_keep = 0;  // summary bits whose words still hold bits after the releases
+  for (uint32_t smi = sm; smi;) {
+    const uint32_t b = (uint32_t)__builtin_ctz(smi);
+    smi &= smi - 1;
+    const uint32_t wend = min((b + 1) * G, nmw);
+    uint32_t any = 0;
+    for (uint32_t w = b * G; w < wend; ++w) {
+      const uint32_t m0 = mrow[w * 64];
+      uint32_t m = m0, bits = m0;
       while (bits) {
         // up to four records fetched together, applied in slot order
-        uint32_t jj[4];
+        uint32_t jj[4], kb[4];
         uint32_t cnt = 0;
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
-          const uint32_t k = 32u * (w0 + u) + (bits ? (uint32_t)__builtin_ctz(bits) : 0u);
-          jj[e] = nc_slot_of(rec.slot_off, lane, k);
+          kb[e] = bits ? (uint32_t)__builtin_ctz(bits) : 0u;
+          jj[e] = nc_slot_of(rec.slot_off, lane, 32u * w + kb[e]);
           if (bits) {
             bits &= bits - 1;
             cnt = e + 1;
@@ -268,23 +288,19 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
             a.slots[jj[e]].last_arr = arr;
             ++n_arr;
           }
-          if (o >= a.r_rel) {  // latch release (:697)
-            const uint32_t rel = jj[e] - rec.slot_off;
-            const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
-#pragma unroll
-            for (int u2 = 0; u2 < NC_MW; ++u2)
-              if (w0 + u2 == (k >> 5)) m[u2] &= ~(1u << (k & 31));
-          }
+          if (o >= a.r_rel) m &= ~(1u << kb[e]);  // latch release (:697)
         }
         n_active += cnt;
       }
-    }
-#pragma unroll
-    for (int u = 0; u < NC_MW; ++u)
-      if (m[u] != m0[u]) {
-        mrow[(w0 + u) * 64] = m[u];
+      if (m != m0) {
+        // clear the released bits; arrival bits are OR-ed in below, so both
+        // are read-modify-writes and commute
+        __hip_atomic_fetch_and(&mrow[w * 64], ~(m0 & ~m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         ++n_mwb;
       }
+      any |= m;
+    }
+    if (any) sm_keep |= 1u << b;
   }
 
   // charge_passive (src/NeuCor.cpp:677-680).
@@ -339,13 +355,11 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   const unsigned long long fm = __ballot(fire);
   if (wave_in && lane == 0)
     a.fired[wave] = live ? make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q) : make_uint4(0u, 0u, 0u, 0u);
-  // The passes below touch slot records and mask words this wave wrote above,
-  // from other lanes: make those writes visible first.
-  if (fm || pend_n) {
+  if (fm) {
+    // The row passes below read slot records this wave wrote above (arrival
+    // STDP), from other lanes: make those writes visible first.
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-  }
-  if (fm) {
     // Post-side STDP of each fired lane (the inSynapses loop of Neuron::fire,
     // src/NeuCor.cpp:651-653): target trace powf(decay, 0) == 1 -> 0, synapse
     // trace of lastSpikeArrival from before this step's arrivals.  The whole
@@ -375,20 +389,26 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   // Spikes of this slice arriving at s leave the pending list and set their
   // slot's active bit (contributing from s+1).  The list is compacted in
   // place (write index <= read index).
+  // This wave's summary words, merged with the arrival bits below.
+  __shared__ uint32_t sh_sum[4][64];
+  const uint32_t wl = threadIdx.x >> 6;
+  sh_sum[wl][lane] = sm_keep;
   if (pend_n) {
-    uint2* pl = a.pend + rec.slot_off;
     uint32_t kept = 0;
     for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
       const uint32_t t = t0 + lane;
-      uint2 e = make_uint2(0u, 0u);
-      if (t < pend_n) e = pl[t];
+      uint2 e = pe0;
+      if (t0 && t < pend_n) e = pl[t];
       const bool arrive = t < pend_n && (int32_t)e.y == s;
+      const uint32_t rel = e.x - rec.slot_off;
+      const uint32_t tl = (rel & 255u) >> 2;
+      const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
+      const uint32_t Gt = (uint32_t)__shfl((int)G, (int)tl);
       if (arrive) {
-        const uint32_t rel = e.x - rec.slot_off;
-        const uint32_t tl = (rel & 255u) >> 2;
-        const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
         __hip_atomic_fetch_or(a.amask + rec.mask_off + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t sb = Gt == 1u ? (k >> 5) : (k >> 5) / Gt;
+        __hip_atomic_fetch_or(&sh_sum[wl][tl], 1u << sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
       }
       const bool keep = t < pend_n && !arrive;
       const unsigned long long km = __ballot(keep);
@@ -398,6 +418,9 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
     }
     if (lane == 0) a.pend_cnt[q] = kept;
   }
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
+  const uint32_t sm_new = sh_sum[wl][lane];
+  if (live && sm_new != sm) a.asum[pos] = sm_new;
 
   if (a.raster_cap) {
     const uint32_t rt = nc_wave_append(fire, a.raster_cnt);
@@ -439,26 +462,52 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
 __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
   const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   const uint32_t lane = threadIdx.x & 63;
-  uint32_t n_out = 0, n_acc = 0;
+  uint32_t n_acc = 0;
   const uint4 f = wave < a.n_active_slices ? a.fired[wave] : make_uint4(0u, 0u, 0u, 0u);
-  unsigned long long m = ((unsigned long long)f.y << 32) | f.x;
-  const int32_t s = (int32_t)f.z;
-  while (m) {
-    const uint32_t l = (uint32_t)__builtin_ctzll(m);
-    m &= m - 1;
-    const uint2 row = a.out_row[f.w * 64 + l];
-    n_out += (lane < row.y) ? ((row.y - lane + 63) >> 6) : 0u;
-    for (uint32_t k = lane; k < row.y; k += 128) {
-      const bool two = k + 64 < row.y;
-      const nc_edge e0 = a.edges[row.x + k];
-      nc_edge e1 = e0;
-      if (two) e1 = a.edges[row.x + k + 64];
-      nc_fire_edge(a, row.x + k, e0, s, n_acc);
-      if (two) nc_fire_edge(a, row.x + k + 64, e1, s, n_acc);
+  const unsigned long long m = ((unsigned long long)f.y << 32) | f.x;
+  uint32_t total = 0;
+  if (m) {
+    // The out-lists of all fired lanes, flattened: lane-parallel row loads,
+    // a wave prefix sum of out-degrees, then 128 edges per pass.
+    const int32_t s = (int32_t)f.z;
+    uint2 row = make_uint2(0u, 0u);
+    if ((m >> lane) & 1ull) row = a.out_row[f.w * 64 + lane];
+    uint32_t incl = row.y;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+      if (lane >= (uint32_t)o) incl += v;
+    }
+    total = (uint32_t)__shfl((int)incl, 63);
+    for (uint32_t base = 0; base < total; base += 128) {
+      uint32_t ie[2], src[2];
+      nc_edge ed[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t i = base + 64u * h + lane;
+        // source lane: the first lane whose inclusive prefix exceeds i
+        uint32_t lo = 0;
+#pragma unroll
+        for (uint32_t b = 32; b >= 1; b >>= 1) {
+          const uint32_t v = (uint32_t)__shfl((int)incl, (int)(lo + b - 1));
+          if (v <= i) lo += b;
+        }
+        src[h] = lo > 63 ? 63 : lo;
+        const uint32_t rx = (uint32_t)__shfl((int)row.x, (int)src[h]);
+        const uint32_t ry = (uint32_t)__shfl((int)row.y, (int)src[h]);
+        const uint32_t ic = (uint32_t)__shfl((int)incl, (int)src[h]);
+        ie[h] = i < total ? rx + (i - (ic - ry)) : ~0u;
+      }
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (ie[h] != ~0u) ed[h] = a.edges[ie[h]];
+#pragma unroll
+      for (int h = 0; h < 2; ++h)
+        if (ie[h] != ~0u) nc_fire_edge(a, ie[h], ed[h], s, n_acc);
     }
   }
   if (a.ctr) {
-    nc_count32(a.ctr, 6, n_out);
+    if (lane == 0 && total) atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 6], (unsigned long long)total);
     nc_count32(a.ctr, 7, n_acc);
   }
 }

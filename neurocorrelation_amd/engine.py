@@ -195,11 +195,14 @@ class Simulation:
                 "mask_words_read", "mask_words_written"]
         return dict(zip(keys, (int(x) for x in out)))
 
+    def set_counting(self, enable):
+        check(lib().nc_set_counting(self._h, 1 if enable else 0))
+
     def synchronize(self):
         check(lib().nc_sim_synchronize(self._h))
 
-    def set_kernel_timing(self, kernel_mask):
-        check(lib().nc_set_kernel_timing(self._h, int(kernel_mask)))
+    def set_kernel_timing(self, kernel_mask, every=1):
+        check(lib().nc_set_kernel_timing(self._h, int(kernel_mask), int(every)))
 
     def kernel_timing(self, kernel):
         n, ms = C.c_uint64(), C.c_double()
