@@ -4,6 +4,7 @@ TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
 CPU-baseline leg of bench.py, as the checker.  The product never imports it.
 """
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -177,6 +178,29 @@ class Sim:
         if getattr(self, "_h", None):
             lib().or_sim_free(self._h)
             self._h = None
+
+
+def drive_schedule(ids, groups, n_corr_groups, period, p0, amp_corr, amp_ind, seed, steps):
+    """Per step 1..steps, the sorted ids of input neurons whose group fires:
+    one Philox draw per (step, group) (stream 4) against floor(p * 2^32),
+    p = p0 * (1 + amp * sin(2 pi (step mod period) / period)), amp_corr for
+    groups below n_corr_groups, amp_ind for the others."""
+    key = (seed & 0xFFFFFFFF, seed >> 32)
+    members = {}
+    for i, g in zip(list(ids), list(groups)):
+        members.setdefault(int(g), []).append(int(i))
+    per = []
+    for s in range(1, steps + 1):
+        k = s % period
+        fired = []
+        for g, mem in members.items():
+            amp = amp_corr if g < n_corr_groups else amp_ind
+            p = p0 * (1.0 + amp * math.sin(2.0 * math.pi * k / period))
+            thr = 0 if p <= 0 else (0xFFFFFFFF if p >= 1 else int(math.floor(p * 4294967296.0)))
+            if int(philox((s, g, 4, 0), key)[0]) < thr:
+                fired.extend(mem)
+        per.append(sorted(fired))
+    return per
 
 
 def trace(decay, x):
