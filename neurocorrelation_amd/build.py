@@ -54,7 +54,24 @@ def build(verbose=False):
     tmp = OUT + ".tmp"
     _run([hipcc, "--offload-arch=" + ARCH, "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, OUT)
+    build_facade()
     return OUT
+
+
+def build_facade():
+    """class NeuCor over the C ABI: libneucor_facade.so and its test driver
+    facade/facade_check, both linked against libneucor_amd.so by rpath."""
+    fdir = os.path.join(HERE, "facade")
+    obj = os.path.join(HERE, "_build", "NeuCor.o")
+    _run(["g++"] + CXX_FLAGS + ["-c", os.path.join(fdir, "NeuCor.cpp"), "-o", obj])
+    link = ["-L" + HERE, "-lneucor_amd", "-Wl,-rpath,$ORIGIN", "-pthread"]
+    tmp = os.path.join(HERE, "libneucor_facade.so.tmp")
+    _run(["g++", "-shared", "-o", tmp, obj] + link)
+    os.replace(tmp, os.path.join(HERE, "libneucor_facade.so"))
+    exe = os.path.join(fdir, "facade_check")
+    _run(["g++"] + CXX_FLAGS + ["-o", exe + ".tmp", os.path.join(fdir, "facade_check.cpp"), obj,
+                                "-L" + HERE, "-lneucor_amd", "-Wl,-rpath,$ORIGIN/..", "-pthread"])
+    os.replace(exe + ".tmp", exe)
 
 
 if __name__ == "__main__":

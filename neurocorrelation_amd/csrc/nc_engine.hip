@@ -413,9 +413,15 @@ hipEvent_t get_event(nc_sim* sim, size_t i) {
 
 extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64_t* bytes) {
   if (!bytes) return fail(NC_ERR_ARG, "null argument");
-  uint64_t npos = (n_neurons + 63) / 64 * 64;
-  uint64_t slots = n_syn + n_syn / 4 + npos * 4;  // padding allowance
-  *bytes = npos * 48 + slots * 17 + n_syn * 5;
+  // the buffers nc_sim_create allocates for one network of this size
+  const uint64_t P = (n_neurons + 63) / 64 * 64 + 64, Q = P / 64;
+  const uint64_t slots = (n_syn + n_syn / 4 + P * 4 + 255) / 256 * 256;  // padding allowance
+  const uint64_t mwords = slots / 32 + 64 * Q;
+  *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
+           + Q * (4 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
+           + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
+           + (n_syn + 64) * sizeof(nc_edge)                // out-edges
+           + (Q / 4 + 2) * NC_NCTR * 8;                    // counters
   return NC_OK;
 }
 

@@ -1,0 +1,64 @@
+// Command-line driver of class NeuCor (this directory) for the tests.
+//
+//   facade_check wiring N SEED
+//       srand(SEED); NeuCor(N); prints the network (no GPU needed):
+//       "n N", then "p x y z" per neuron, then "s from to weight length"
+//       per synapse in creation order (floats as C99 hex).
+//   facade_check few RUNS
+//       the FEW_NEURONS preset of src/main.cpp:162-189 (3 neurons, 0 -> 1 and
+//       0 -> 2 at w = 0.5, inputs at 50 Hz with offsets +2 / -2 ms,
+//       runAll, runSpeed 0.02 ms) for RUNS calls of run(); prints
+//       "t TIME", "dt STEP_MS", "w FROM TO W" per synapse and "v ID V" per
+//       neuron (hex).
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <exception>
+
+#include "NeuCor.h"
+
+static int wiring(int n, unsigned seed) {
+  srand(seed);
+  NeuCor brain(n);
+  printf("n %zu\n", brain.getNeuronCount());
+  for (const auto& s : brain.getNeuronSnapshots())
+    printf("p %a %a %a\n", s.position.x, s.position.y, s.position.z);
+  for (const auto& s : brain.getSynapseSnapshots())
+    printf("s %zu %zu %a %a\n", s.fromID, s.toID, s.weight, s.from.getDist(s.to));
+  return 0;
+}
+
+static int few(long runs) {
+  NeuCor brain(0);
+  brain.runAll = true;
+  brain.runSpeed = 0.02f;
+  const coord3 pos[3] = {{0, 0, 0}, {0.3f, 0.3f, 0}, {0.3f, -0.3f, 0}};
+  for (const coord3& p : pos) brain.createNeuron(p);
+  brain.createSynapse(1, 0, 0.5f);
+  brain.createSynapse(2, 0, 0.5f);
+  float inputs[3] = {50.0f, 50.0f, 50.0f};
+  coord3 ipos[3] = {pos[0], pos[1], pos[2]};
+  float irad[3] = {0.1f, 0.1f, 0.1f};
+  brain.setInputRateArray(inputs, 3, ipos, irad);
+  brain.addInputOffset(1, 2.0f);
+  brain.addInputOffset(2, -2.0f);
+  for (long k = 0; k < runs; ++k) brain.run();
+  printf("t %a\n", brain.getTime());
+  printf("dt %a\n", brain.stepMs);
+  for (const auto& s : brain.getSynapseSnapshots()) printf("w %zu %zu %a\n", s.fromID, s.toID, s.weight);
+  for (const auto& s : brain.getNeuronSnapshots()) printf("v %zu %a\n", s.id, s.potential);
+  return 0;
+}
+
+int main(int argc, char** argv) {
+  try {
+    if (argc == 4 && strcmp(argv[1], "wiring") == 0) return wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
+    if (argc == 3 && strcmp(argv[1], "few") == 0) return few(atol(argv[2]));
+  } catch (const std::exception& e) {
+    fprintf(stderr, "facade_check: %s\n", e.what());
+    return 1;
+  }
+  fprintf(stderr, "usage: facade_check wiring N SEED | few RUNS\n");
+  return 2;
+}

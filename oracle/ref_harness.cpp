@@ -40,13 +40,15 @@
 // ---- scripted rand() -------------------------------------------------------
 static std::deque<int> g_script;
 static bool g_bench = false;
+static bool g_libc_rand = false;  // --wiring: the C library's own stream
 typedef int (*rand_fn)(void);
 static rand_fn g_real_rand = nullptr;
 
 extern "C" int rand(void) {
-  if (g_bench) {
+  if (g_bench || g_libc_rand) {
     if (!g_real_rand) g_real_rand = (rand_fn)dlsym(RTLD_NEXT, "rand");
-    (void)g_real_rand();
+    const int v = g_real_rand();
+    if (g_libc_rand) return v;
   }
   if (g_script.empty()) return 1;  // 1 % period != 0: no background fire
   int v = g_script.front();
@@ -61,7 +63,25 @@ class NeuCor_Renderer {
   static void set_length(Synapse& s, float L) { s.length = L; }
   static float fire_t(const Synapse& s) { return s.AP_fireTime; }
   static float last_arr(const Synapse& s) { return s.lastSpikeArrival; }
+  static float length(const Synapse& s) { return s.length; }
+  static size_t target(const Synapse& s) { return s.tN; }
 };
+
+// --wiring N SEED: the network NeuCor(N) builds after srand(SEED), printed as
+// "n N", "p x y z" per neuron and "s from to weight length" per synapse in
+// creation order (floats as C99 hex).
+static int dump_wiring(int n, unsigned seed) {
+  g_libc_rand = true;
+  srand(seed);
+  NeuCor brain(n);
+  std::deque<Neuron>& neu = NeuCor_Renderer::neurons(brain);
+  printf("n %zu\n", brain.getNeuronCount());
+  for (const auto& s : brain.getNeuronSnapshots()) printf("p %a %a %a\n", s.position.x, s.position.y, s.position.z);
+  for (auto& nn : neu)
+    for (auto& sy : nn.outSynapses)
+      printf("s %zu %zu %a %a\n", nn.getID(), NeuCor_Renderer::target(sy), sy.getWeight(), NeuCor_Renderer::length(sy));
+  return 0;
+}
 
 struct Scenario {
   uint32_t n = 0, steps = 0, flags = 0;
@@ -111,6 +131,7 @@ int main(int argc, char** argv) {
     fprintf(stderr, "usage: ref_harness [--bench] scenario.bin out.bin\n");
     return 2;
   }
+  if (argc == 4 && strcmp(argv[1], "--wiring") == 0) return dump_wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
   int a = 1;
   if (strcmp(argv[a], "--bench") == 0) {
     g_bench = true;
