@@ -31,6 +31,25 @@ void check(nc_status st, const char* what) {
   }
 }
 
+// The HIP runtime draws from the C library's rand() while it sets a device
+// up, and the reference's own draws (construction, input positions,
+// background fires) and its callers' share that process-wide stream.  Every
+// engine call runs with a scratch rand() state installed, so the caller's
+// stream advances exactly as it would with the reference (setstate keeps the
+// position of a swapped-out state in the state array itself).
+class CallerRandStream {
+ public:
+  CallerRandStream() { prev_ = initstate(1u, scratch_, sizeof scratch_); }
+  ~CallerRandStream() { setstate(prev_); }
+  CallerRandStream(const CallerRandStream&) = delete;
+  CallerRandStream& operator=(const CallerRandStream&) = delete;
+
+ private:
+  static char scratch_[128];
+  char* prev_;
+};
+char CallerRandStream::scratch_[128];
+
 // Synapse::getPrePot / getPostPot's display curve (src/NeuCor.cpp:528-532).
 float renderPulse(float val) {
   val = fmin(fmax(val, 0.0), 0.7);
@@ -89,6 +108,7 @@ NeuCor::NeuCor(int n_neurons)
 }
 
 NeuCor::~NeuCor() {
+  CallerRandStream keep;
   if (sim) nc_sim_destroy(sim);
   if (net) nc_net_free(net);
 }
@@ -187,6 +207,7 @@ float NeuCor::getTime() const { return currentTime; }
 
 void NeuCor::pullWeights() {
   if (!sim) return;
+  CallerRandStream keep;
   std::vector<float> w(delaySteps.size());
   if (!w.empty()) check(nc_read_weights(sim, netId, w.data(), w.size()), "nc_read_weights");
   // synapses created since the engine network was built keep their weights
@@ -197,6 +218,7 @@ void NeuCor::pullWeights() {
 
 void NeuCor::upload() {
   pullWeights();  // a rebuild keeps the learned weights
+  CallerRandStream keep;
   if (sim) {
     nc_sim_destroy(sim);
     sim = nullptr;
@@ -284,6 +306,7 @@ void NeuCor::upload() {
 void NeuCor::pullState() {
   const std::size_t n = positions.size();
   if (!sim || n == 0) return;
+  CallerRandStream keep;
   std::vector<float> V(n);
   std::vector<uint32_t> fi(n);
   check(nc_read_potential(sim, netId, V.data(), (uint32_t)n), "nc_read_potential");
@@ -296,7 +319,10 @@ void NeuCor::pullState() {
 }
 
 void NeuCor::resetActivities() {
-  if (sim) check(nc_reset_activities(sim, netId), "nc_reset_activities");
+  if (sim) {
+    CallerRandStream keep;
+    check(nc_reset_activities(sim, netId), "nc_reset_activities");
+  }
   for (std::size_t i = 0; i < positions.size(); ++i) {
     activityStart[i] = currentTime;
     potAct[2 * i + 1] = 0.0f;
@@ -358,6 +384,7 @@ void NeuCor::run() {
   const uint64_t target = (uint64_t)std::floor((double)targetTime / stepMs);
   const uint64_t steps = target > stepsDone ? target - stepsDone : 0;
   if (sim) {
+    CallerRandStream keep;
     if (learningRate != setLr || presynapticFactor != setPre || postsynapticFactor != setPost) {
       check(nc_set_params(sim, netId, learningRate, presynapticFactor, postsynapticFactor), "nc_set_params");
       setLr = learningRate, setPre = presynapticFactor, setPost = postsynapticFactor;
@@ -388,6 +415,7 @@ std::vector<NeuCor::SynapseSnapshot> NeuCor::getSynapseSnapshots() const {
   std::vector<int32_t> arrival(S), lastArr(S);
   const bool live = sim && !dirty && S;
   if (live) {
+    CallerRandStream keep;
     check(nc_read_weights(sim, netId, w.data(), S), "nc_read_weights");
     check(nc_read_synapse_state(sim, netId, arrival.data(), lastArr.data(), depol.data(), S),
           "nc_read_synapse_state");

@@ -8,8 +8,8 @@
 //       the FEW_NEURONS preset of src/main.cpp:162-189 (3 neurons, 0 -> 1 and
 //       0 -> 2 at w = 0.5, inputs at 50 Hz with offsets +2 / -2 ms,
 //       runAll, runSpeed 0.02 ms) for RUNS calls of run(); prints
-//       "t TIME", "dt STEP_MS", "w FROM TO W" per synapse and "v ID V" per
-//       neuron (hex).
+//       "t TIME", "dt STEP_MS", "w FROM TO W" per synapse, "v ID V" per
+//       neuron (hex) and "r R", the next value of the caller's rand().
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -48,6 +48,7 @@ static int few(long runs) {
   printf("dt %a\n", brain.stepMs);
   for (const auto& s : brain.getSynapseSnapshots()) printf("w %zu %zu %a\n", s.fromID, s.toID, s.weight);
   for (const auto& s : brain.getNeuronSnapshots()) printf("v %zu %a\n", s.id, s.potential);
+  printf("r %d\n", rand());  // the caller's rand() stream after the run
   return 0;
 }
 

@@ -72,6 +72,8 @@ def _few_schedule(runs, dt):
     """Forced fires (1-based step, id, kind) of the FEW_NEURONS facade run."""
     libc = ctypes.CDLL("libc.so.6")
     libc.srand(1)  # the C library's stream before any srand() call
+    for _ in range(2 * 2):  # createSynapse draws a weight and a sign per synapse (src/NeuCor.cpp:471-473)
+        libc.rand()
     RAND_MAX = 2147483647
     f32 = np.float32
     cur, rs = f32(0.0), f32(0.02)
@@ -93,7 +95,7 @@ def _few_schedule(runs, dt):
                 off = f32(f32(libc.rand()) / f32(RAND_MAX)) * rs
                 ev.append((int(math.ceil((np.float64(cur) + np.float64(f32(off))) / dt)), nid, 1))
         cur = f32(cur + rs)
-    return ev, cur
+    return ev, cur, libc.rand()
 
 
 @pytest.mark.gpu
@@ -102,7 +104,7 @@ def test_few_neurons_through_the_facade(oracle_mod):
     print("facade_check few %d" % runs, flush=True)
     out = subprocess.run([_check_bin(), "few", str(runs)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
-    t = dt = None
+    t = dt = r_next = None
     w, v = {}, {}
     for line in out.stdout.splitlines():
         f = line.split()
@@ -114,9 +116,13 @@ def test_few_neurons_through_the_facade(oracle_mod):
             w[(int(f[1]), int(f[2]))] = np.float32(float.fromhex(f[3]))
         elif f[0] == "v":
             v[int(f[1])] = np.float32(float.fromhex(f[2]))
+        elif f[0] == "r":
+            r_next = int(f[1])
     assert dt == 1.0 / 64
-    ev, t_expect = _few_schedule(runs, dt)
+    ev, t_expect, r_expect = _few_schedule(runs, dt)
     assert np.float32(t) == t_expect
+    # the engine left the caller's rand() stream where the reference would
+    assert r_next == r_expect
     steps = int(math.floor(t / dt))
 
     # the network: 0 -> 1, 0 -> 2 at w = 0.5; delay 2 * length ms in steps
@@ -133,5 +139,6 @@ def test_few_neurons_through_the_facade(oracle_mod):
     assert w[(0, 1)] == st["w"][0] and w[(0, 2)] == st["w"][1]
     assert all(v[i] == st["V"][i] for i in range(3))
     # the essay's outcome: the synapse whose target is driven after its
-    # source strengthens to 1, the one driven before weakens to 0
-    assert w[(0, 1)] == pytest.approx(1.0) and w[(0, 2)] == pytest.approx(0.0)
+    # source strengthens to 1, the one driven before weakens to about 0
+    # (each new pre-before-post pairing and background fire moves it a little)
+    assert w[(0, 1)] == pytest.approx(1.0) and 0.0 <= w[(0, 2)] < 0.05
