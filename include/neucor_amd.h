@@ -244,6 +244,27 @@ nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
  * kernels then carry no instrumentation). */
 nc_status nc_set_counting(nc_sim* sim, int enable);
 
+/* Launch shape of the step kernels.  No counterpart in the reference: the
+ * knobs of the batched form, for callers that pack many networks into one
+ * nc_sim.  waves_per_block: 1, 2, 4 or 8 slices (64 neurons each) per
+ * workgroup; xcd_swizzle: each of the 8 XCDs takes one contiguous range of
+ * slices; graph_steps: steps per captured hipGraph chunk (0 = direct
+ * launches; a chunk is replayed while the set of networks is unchanged).
+ * Results do not depend on the launch shape.  Default {4, 0, 0}. */
+typedef struct nc_launch_cfg {
+  uint32_t waves_per_block;
+  uint32_t xcd_swizzle;
+  uint32_t graph_steps;
+} nc_launch_cfg;
+nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg);
+nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg);
+
+/* Autotuner of the launch shape: runs steps_per_trial ordinary steps of every
+ * active network under each candidate shape (after a warm-up of the same
+ * length, which also captures graphs), keeps the fastest by device time and
+ * reports it in *chosen.  The steps count as simulated steps. */
+nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen);
+
 /* Block until every step enqueued on the nc_sim's stream has finished. */
 nc_status nc_sim_synchronize(nc_sim* sim);
 

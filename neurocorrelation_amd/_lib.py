@@ -48,6 +48,10 @@ class DriveSpec(C.Structure):
     ]
 
 
+class LaunchCfg(C.Structure):
+    _fields_ = [("waves_per_block", C.c_uint32), ("xcd_swizzle", C.c_uint32), ("graph_steps", C.c_uint32)]
+
+
 class SimSpec(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("max_neurons", C.c_uint64), ("max_synapses", C.c_uint64),
@@ -89,6 +93,9 @@ SIGNATURES = {
     "nc_reset_activities": (C.c_int, [P, C.c_int32]),
     "nc_read_counters": (C.c_int, [P, u64p, C.c_uint32]),
     "nc_set_counting": (C.c_int, [P, C.c_int]),
+    "nc_sim_set_launch": (C.c_int, [P, C.POINTER(LaunchCfg)]),
+    "nc_sim_get_launch": (C.c_int, [P, C.POINTER(LaunchCfg)]),
+    "nc_sim_tune": (C.c_int, [P, C.c_uint32, C.POINTER(LaunchCfg)]),
     "nc_sim_synchronize": (C.c_int, [P]),
     "nc_set_kernel_timing": (C.c_int, [P, C.c_int, C.c_uint32]),
     "nc_read_kernel_timing": (C.c_int, [P, C.c_uint32, u64p, C.POINTER(C.c_double)]),

@@ -260,6 +260,12 @@ struct nc_sim {
   uint32_t *raster = nullptr, *raster_cnt = nullptr;
   unsigned long long* ctr = nullptr;
   bool counting = false;
+  uint32_t* d_chunk_off = nullptr;  // steps of the current call before the running chunk
+  nc_launch_cfg cfg{4, 0, 0};
+  // captured chunk of cfg.graph_steps steps and the launch it was captured for
+  hipGraphExec_t graph = nullptr;
+  nc_step_args graph_args{};
+  uint32_t graph_key[4] = {0, 0, 0, 0};
   nc_netdesc* d_nets = nullptr;
   float *t_pre = nullptr, *t_post = nullptr, *w_ap = nullptr;
   DevBuf<nc_group> d_groups;
@@ -311,7 +317,8 @@ nc_status grow(DevBuf<T>* b, size_t n, hipStream_t st) {
 }
 
 nc_step_args base_args(nc_sim* sim) {
-  nc_step_args a{};
+  nc_step_args a;
+  memset(&a, 0, sizeof a);  // padding too: launch arguments are compared bytewise
   a.slots = sim->slots;
   a.amask = sim->amask;
   a.asum = sim->asum;
@@ -339,7 +346,9 @@ nc_step_args base_args(nc_sim* sim) {
   a.w_ap = sim->w_ap;
   a.len_pre = (int32_t)sim->T.t_pre.size();
   a.len_post = (int32_t)sim->T.t_post.size();
-  a.tabs_in_lds = (a.len_pre <= NC_LDS_TAB && a.len_post <= NC_LDS_TAB) ? 1 : 0;
+  a.tabs_in_lds = (uint64_t)(a.len_pre + a.len_post) * 4 <= NC_LDS_TAB_BYTES ? 1 : 0;
+  a.chunk_off = sim->d_chunk_off;
+  a.xcd_swizzle = sim->cfg.xcd_swizzle;
   for (int k = 0; k < NC_WAP_K; ++k)
     a.w_ap_k[k] = k < (int)sim->T.w_ap.size() ? sim->T.w_ap[k] : 0.0f;
   a.dt = sim->T.dt;
@@ -421,7 +430,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
            + Q * (4 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
            + (n_syn + 64) * sizeof(nc_edge)                // out-edges
-           + (Q / 4 + 2) * NC_NCTR * 8;                    // counters
+           + (Q + 2) * NC_NCTR * 8;                        // counters
   return NC_OK;
 }
 
@@ -486,7 +495,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
 
     H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
     if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
-    sim->ctr_rows = Q / 4 + 2;
+    sim->ctr_rows = Q + 2;
+    H(hipMalloc(&sim->d_chunk_off, 256), "hipMalloc chunk_off");
     H(hipMalloc(&sim->ctr, sim->ctr_rows * NC_NCTR * 8), "hipMalloc ctr");
     H(hipMalloc(&sim->d_nets, sim->max_nets * sizeof(nc_netdesc)), "hipMalloc nets");
   }
@@ -523,7 +533,8 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (!sim) return;
   (void)hipSetDevice(sim->device);
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
-  void* ptrs[] = {sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
+  if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
+  void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
                   sim->nid, sim->out_row, sim->edges, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
@@ -924,44 +935,94 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     hipLaunchKernelGGL(k_groups, dim3((uint32_t)((nthreads + 255) / 256)), dim3(256), 0, st, a, max_run);
     NC_HIP(hipGetLastError());
   }
-  const uint32_t blocks1 = (a.n_active_slices + 3) / 4;
+  const uint32_t wpb = sim->cfg.waves_per_block;
+  const uint32_t grid = (a.n_active_slices + wpb - 1) / wpb;
+  const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
+  const void* fn_neuron = nullptr;
+  const void* fn_fanout = nullptr;
+  switch (wpb) {
+    case 1: fn_neuron = (const void*)k_neuron<1>, fn_fanout = (const void*)k_fanout<1>; break;
+    case 2: fn_neuron = (const void*)k_neuron<2>, fn_fanout = (const void*)k_fanout<2>; break;
+    case 8: fn_neuron = (const void*)k_neuron<8>, fn_fanout = (const void*)k_fanout<8>; break;
+    default: fn_neuron = (const void*)k_neuron<4>, fn_fanout = (const void*)k_fanout<4>; break;
+  }
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, sim->d_chunk_off, 0u);
+  NC_HIP(hipGetLastError());
   size_t nev_used = 0;
   std::vector<int> timed_kinds;
-  for (uint32_t i = 0; i < max_run; ++i) {
-    a.chunk_i = (int32_t)i;
+  auto launch = [&](int kind, const void* fn, uint32_t g, uint32_t block, uint32_t shmem) -> nc_status {
+    if (g == 0) return NC_OK;
+    void* args[] = {&a};
+    const bool timed = ((sim->kernel_timing >> kind) & 1) && (sim->timing_step % sim->timing_every) == 0;
+    if (timed) {
+      // start/stop events carried by the dispatch itself: they time the
+      // kernel, not the queue around it
+      hipEvent_t e0 = get_event(sim, nev_used++);
+      hipEvent_t e1 = get_event(sim, nev_used++);
+      if (!e0 || !e1) return fail(NC_ERR_HIP, "hipEventCreate failed");
+      NC_HIP(hipExtLaunchKernel(fn, dim3(g), dim3(block), args, shmem, st, e0, e1, 0));
+      sim->k_launch[kind]++;
+      timed_kinds.push_back(kind);
+    } else {
+      NC_HIP(hipLaunchKernel(fn, dim3(g), dim3(block), args, shmem, st));
+    }
+    return NC_OK;
+  };
+  // Whole chunks as captured graphs when nothing per-step comes from the host.
+  uint32_t done = 0;
+  const uint32_t G = sim->cfg.graph_steps;
+  if (G && ev_pos.empty() && sim->kernel_timing == 0 && max_run >= G) {
+    a.chunk_i = 0;
+    const uint32_t key[4] = {G, wpb, grid, lds};
+    if (!sim->graph || memcmp(&sim->graph_args, &a, sizeof a) != 0 || memcmp(sim->graph_key, key, sizeof key) != 0) {
+      if (sim->graph) {
+        (void)hipGraphExecDestroy(sim->graph);
+        sim->graph = nullptr;
+      }
+      NC_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      nc_status cs = NC_OK;
+      for (uint32_t c = 0; c < G && cs == NC_OK; ++c) {
+        a.chunk_i = (int32_t)c;
+        void* args[] = {&a};
+        if (hipLaunchKernel(fn_neuron, dim3(grid), dim3(64 * wpb), args, lds, st) != hipSuccess ||
+            hipLaunchKernel(fn_fanout, dim3(grid), dim3(64 * wpb), args, 0, st) != hipSuccess)
+          cs = fail(NC_ERR_HIP, "kernel launch during capture failed");
+      }
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, sim->d_chunk_off, G);
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(st, &g);
+      if (cs != NC_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return cs;
+      }
+      NC_HIP(ce);
+      const hipError_t ie = hipGraphInstantiate(&sim->graph, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      NC_HIP(ie);
+      a.chunk_i = 0;
+      sim->graph_args = a;
+      memcpy(sim->graph_key, key, sizeof key);
+    }
+    for (; max_run - done >= G; done += G) NC_HIP(hipGraphLaunch(sim->graph, st));
+  }
+  for (uint32_t i = done; i < max_run; ++i) {
+    a.chunk_i = (int32_t)(i - done);  // chunk_off holds `done` on the device
     a.n_ev = ev_ptr[i + 1] - ev_ptr[i];
     a.ev_pos = sim->d_ev_pos.p + ev_ptr[i];
     a.ev_kind = sim->d_ev_kind.p + ev_ptr[i];
     a.use_forced = ev_pos.empty() ? 0 : 1;
-    auto launch = [&](int kind, auto fn, uint32_t grid) -> nc_status {
-      if (grid == 0) return NC_OK;
-      hipEvent_t e0 = nullptr, e1 = nullptr;
-      const bool timed = ((sim->kernel_timing >> kind) & 1) && (sim->timing_step % sim->timing_every) == 0;
-      if (timed) {
-        // start/stop events carried by the dispatch itself: they time the
-        // kernel, not the queue around it
-        e0 = get_event(sim, nev_used++);
-        e1 = get_event(sim, nev_used++);
-        if (!e0 || !e1) return fail(NC_ERR_HIP, "hipEventCreate failed");
-        hipExtLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, e0, e1, 0, a);
-      } else {
-        hipLaunchKernelGGL(fn, dim3(grid), dim3(256), 0, st, a);
-      }
-      NC_HIP(hipGetLastError());
-      if (timed) {
-        sim->k_launch[kind]++;
-        timed_kinds.push_back(kind);
-      }
-      return NC_OK;
-    };
     nc_status s1 = NC_OK;
-    if (a.n_ev) s1 = launch(0, k_drive, (a.n_ev + 255) / 256);
-    if (s1 == NC_OK) s1 = launch(1, k_neuron, blocks1);
-    if (s1 == NC_OK) s1 = launch(3, k_fanout, blocks1);
+    if (a.n_ev) {
+      void* args[] = {&a};
+      hipError_t e = hipLaunchKernel((const void*)k_drive, dim3((a.n_ev + 255) / 256), dim3(256), args, 0, st);
+      if (e != hipSuccess) s1 = fail(NC_ERR_HIP, hipGetErrorString(e));
+    }
+    if (s1 == NC_OK) s1 = launch(1, fn_neuron, grid, 64 * wpb, lds);
+    if (s1 == NC_OK) s1 = launch(3, fn_fanout, grid, 64 * wpb, 0);
     if (s1 != NC_OK) return s1;
-    sim->gstep++;
     sim->timing_step++;
   }
+  sim->gstep += max_run;
   if (!timed_kinds.empty()) {
     NC_HIP(hipEventSynchronize(sim->ev_pool[nev_used - 1]));
     for (size_t q = 0; q < timed_kinds.size(); ++q) {
@@ -1207,6 +1268,67 @@ extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) {
   all[9] = sim->host_mword_steps;
   for (uint32_t k = 0; k < n; ++k) out[k] = all[k];
   return NC_OK;
+}
+
+extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) {
+  if (!sim || !cfg) return fail(NC_ERR_ARG, "null argument");
+  const uint32_t w = cfg->waves_per_block;
+  if (w != 1 && w != 2 && w != 4 && w != 8) return fail(NC_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
+  if (cfg->xcd_swizzle > 1) return fail(NC_ERR_ARG, "xcd_swizzle must be 0 or 1");
+  if (cfg->graph_steps > 4096) return fail(NC_ERR_ARG, "graph_steps must be at most 4096");
+  sim->cfg = *cfg;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg) {
+  if (!sim || !cfg) return fail(NC_ERR_ARG, "null argument");
+  *cfg = sim->cfg;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen) {
+  if (!sim || steps_per_trial == 0) return fail(NC_ERR_ARG, "null sim or zero steps");
+  NC_HIP(hipSetDevice(sim->device));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  NC_HIP(hipEventCreate(&e0));
+  if (hipEventCreate(&e1) != hipSuccess) {
+    (void)hipEventDestroy(e0);
+    return fail(NC_ERR_HIP, "hipEventCreate failed");
+  }
+  const nc_launch_cfg start = sim->cfg;
+  nc_launch_cfg best = start;
+  float best_ms = -1.0f;
+  nc_status st = NC_OK;
+  const uint32_t waves[4] = {1, 2, 4, 8};
+  const uint32_t graphs[2] = {0, steps_per_trial};
+  for (uint32_t g : graphs)
+    for (uint32_t w : waves)
+      for (uint32_t x = 0; x < 2 && st == NC_OK; ++x) {
+        sim->cfg = nc_launch_cfg{w, x, g};
+        st = nc_sim_step(sim, steps_per_trial);  // warm-up (and graph capture)
+        if (st != NC_OK) break;
+        if (hipEventRecord(e0, sim->stream) != hipSuccess) {
+          st = fail(NC_ERR_HIP, "hipEventRecord failed");
+          break;
+        }
+        st = nc_sim_step(sim, steps_per_trial);
+        if (st != NC_OK) break;
+        float ms = 0.0f;
+        if (hipEventRecord(e1, sim->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+          st = fail(NC_ERR_HIP, "event timing failed");
+          break;
+        }
+        if (best_ms < 0.0f || ms < best_ms) {
+          best_ms = ms;
+          best = sim->cfg;
+        }
+      }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  sim->cfg = st == NC_OK ? best : start;
+  if (chosen) *chosen = sim->cfg;
+  return st;
 }
 
 extern "C" nc_status nc_set_counting(nc_sim* sim, int enable) {

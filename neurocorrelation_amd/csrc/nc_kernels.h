@@ -25,7 +25,7 @@
 #define NC_DEG_INPUT 0x01000000u
 #define NC_DEG_VALID 0x02000000u
 #define NC_SLICE_NONE 0xFFFFFFFFu
-#define NC_LDS_TAB 2048  // trace-table entries staged in LDS per block
+#define NC_LDS_TAB_BYTES 32768  // trace tables up to this size are staged in LDS
 #define NC_WAP_K 8       // AP-table entries passed as kernel arguments
 #define NC_MW 4          // active-mask words (32 slots each) a lane holds per pass
 #define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
@@ -98,12 +98,14 @@ struct nc_step_args {
   const float* t_post;
   const float* w_ap;
   int32_t len_pre, len_post;
-  int32_t tabs_in_lds;     // both trace tables fit NC_LDS_TAB entries
+  int32_t tabs_in_lds;     // both trace tables are staged in (dynamic) LDS
   float w_ap_k[NC_WAP_K];  // the AP table when it has <= NC_WAP_K entries
   float dt, P, depol_k;
   double c0, e;
   int32_t d_ref, r_rel;
-  int32_t chunk_i;
+  int32_t chunk_i;            // step of the launch inside its chunk
+  const uint32_t* chunk_off;  // device: steps of this call before the chunk
+  uint32_t xcd_swizzle;       // blocks of one XCD take one contiguous range of slices
   const uint2* out_row;     // per position: first out-edge, out-degree
   nc_edge* edges;           // out-edges, rows in ascending target id
   uint2* pend;             // per slice, at its slot range: (slot, arrival step) of spikes in flight
@@ -154,9 +156,29 @@ __device__ __forceinline__ uint32_t nc_wave_append(bool pred, uint32_t* cnt) {
 
 // Adds per-lane counts into the block's own counter row: a wave sum, then
 // one fire-and-forget atomic from lane 0 (a row sees at most 4 writers).
-__device__ __forceinline__ void nc_count32(unsigned long long* ctr, int k, uint32_t v) {
+__device__ __forceinline__ void nc_count32(unsigned long long* ctr, uint32_t row, int k, uint32_t v) {
   for (int o = 32; o > 0; o >>= 1) v += (uint32_t)__shfl_xor((int)v, o);
-  if (nc_lane() == 0 && v) atomicAdd(&ctr[(size_t)blockIdx.x * NC_NCTR + k], (unsigned long long)v);
+  if (nc_lane() == 0 && v) atomicAdd(&ctr[(size_t)row * NC_NCTR + k], (unsigned long long)v);
+}
+
+// Block index after the optional XCD remap: workgroups are dealt round-robin
+// to the 8 XCDs, so block b runs on XCD b % 8; the remap gives each XCD one
+// contiguous range of blocks (and of slices), which keeps a slice range and
+// its spatially close fan-out targets in one L2.
+__device__ __forceinline__ uint32_t nc_block_id(uint32_t swizzle) {
+  const uint32_t b = blockIdx.x;
+  if (!swizzle) return b;
+  const uint32_t nb = gridDim.x, q = nb >> 3, r = nb & 7u, x = b & 7u, i = b >> 3;
+  return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
+
+// Advances the device-side chunk offset by the chunk's step count (the last
+// node of a captured chunk); k_set_u32 clears it at the start of a call.
+__global__ void k_advance(uint32_t* chunk_off, uint32_t steps) {
+  if (threadIdx.x == 0) *chunk_off += steps;
+}
+__global__ void k_set_u32(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0) *p = v;
 }
 
 // Slot index of in-slot k of lane l in a slice whose slots start at slot_off
@@ -197,10 +219,14 @@ __global__ void k_drive(nc_step_args a) {
   atomicOr(&m[pos >> 5], 1u << (pos & 31));
 }
 
-// Neuron::run (src/NeuCor.cpp:619-641) for every neuron of the active slices.
-__global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
-  __shared__ float sh_pre[NC_LDS_TAB], sh_post[NC_LDS_TAB];
-  const uint32_t wave = __builtin_amdgcn_readfirstlane((blockIdx.x * blockDim.x + threadIdx.x) >> 6);
+// Neuron::run (src/NeuCor.cpp:619-641) for every neuron of the active slices;
+// WPB waves (slices) per workgroup.
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
+  extern __shared__ float sh_tab[];  // trace tables: pre then post
+  const uint32_t blk = nc_block_id(a.xcd_swizzle);
+  const uint32_t wl = threadIdx.x >> 6;
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * WPB + wl);
   const uint32_t lane = threadIdx.x & 63;
   const bool wave_in = wave < a.n_active_slices;
   const uint32_t widx = wave_in ? wave : 0u;
@@ -215,7 +241,8 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   uint32_t pend_n = a.pend_cnt[q];
   uint32_t sm = a.asum[pos];
   const nc_netdesc nd = a.nets[rec.net];
-  const bool live = wave_in && a.chunk_i < nd.remaining;
+  const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;  // step index inside this call
+  const bool live = wave_in && ci < nd.remaining;
   if (!live) {
     nw = make_uint2(0u, 0u);
     pend_n = 0u;
@@ -225,18 +252,18 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   uint2* const pl = a.pend + rec.slot_off;
   uint2 pe0 = make_uint2(0u, 0u);
   if (lane < pend_n) pe0 = pl[lane];
-  const int32_t s = nd.base_step + a.chunk_i + 1;
+  const int32_t s = nd.base_step + ci + 1;
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
   const float lr = nd.lr, pre_f = nd.pre_f, post_f = nd.post_f;
   if (a.tabs_in_lds) {
-    for (int32_t i = threadIdx.x; i < a.len_pre; i += blockDim.x) sh_pre[i] = a.t_pre[i];
-    for (int32_t i = threadIdx.x; i < a.len_post; i += blockDim.x) sh_post[i] = a.t_post[i];
+    for (int32_t i = threadIdx.x; i < a.len_pre; i += 64 * WPB) sh_tab[i] = a.t_pre[i];
+    for (int32_t i = threadIdx.x; i < a.len_post; i += 64 * WPB) sh_tab[a.len_pre + i] = a.t_post[i];
+    __syncthreads();
   }
-  __syncthreads();
-  const float* tpre = a.tabs_in_lds ? sh_pre : a.t_pre;
-  const float* tpost = a.tabs_in_lds ? sh_post : a.t_post;
+  const float* tpre = a.tabs_in_lds ? sh_tab : a.t_pre;
+  const float* tpost = a.tabs_in_lds ? sh_tab + a.len_pre : a.t_post;
   uint32_t n_active = 0, n_arr = 0, n_mwb = 0;
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
@@ -310,7 +337,7 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   bool f_in = false, f_bg = false;
   if (valid && (dw & NC_DEG_INPUT)) {
     const uint32_t g = nw.y;
-    f_in = ((a.gfire[(size_t)a.chunk_i * a.gwords + (g >> 5)] >> (g & 31)) & 1u) != 0;
+    f_in = ((a.gfire[(size_t)ci * a.gwords + (g >> 5)] >> (g & 31)) & 1u) != 0;
   }
   if (valid && nd.bg_thr) {
     const uint64_t seed = ((uint64_t)nd.bg_hi << 32) | nd.bg_lo;
@@ -390,8 +417,7 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
   // slot's active bit (contributing from s+1).  The list is compacted in
   // place (write index <= read index).
   // This wave's summary words, merged with the arrival bits below.
-  __shared__ uint32_t sh_sum[4][64];
-  const uint32_t wl = threadIdx.x >> 6;
+  __shared__ uint32_t sh_sum[WPB][64];
   sh_sum[wl][lane] = sm_keep;
   if (pend_n) {
     uint32_t kept = 0;
@@ -430,13 +456,13 @@ __global__ __launch_bounds__(256) void k_neuron(nc_step_args a) {
     }
   }
   if (a.ctr) {
-    nc_count32(a.ctr, 2, n_active);
-    nc_count32(a.ctr, 3, n_arr);
+    nc_count32(a.ctr, blk, 2, n_active);
+    nc_count32(a.ctr, blk, 3, n_arr);
     if (lane == 0 && fm)
-      atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
-    nc_count32(a.ctr, 5, n_post);
-    nc_count32(a.ctr, 10, n_mwb);
-    if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 8], (unsigned long long)pend_n);
+      atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
+    nc_count32(a.ctr, blk, 5, n_post);
+    nc_count32(a.ctr, blk, 10, n_mwb);
+    if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)pend_n);
   }
 }
 
@@ -459,8 +485,10 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   ++n_acc;
 }
 
-__global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
+  const uint32_t blk = nc_block_id(a.xcd_swizzle);
+  const uint32_t wave = blk * WPB + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
   uint32_t n_acc = 0;
   const uint4 f = wave < a.n_active_slices ? a.fired[wave] : make_uint4(0u, 0u, 0u, 0u);
@@ -507,8 +535,8 @@ __global__ __launch_bounds__(256) void k_fanout(nc_step_args a) {
     }
   }
   if (a.ctr) {
-    if (lane == 0 && total) atomicAdd(&a.ctr[(size_t)blockIdx.x * NC_NCTR + 6], (unsigned long long)total);
-    nc_count32(a.ctr, 7, n_acc);
+    if (lane == 0 && total) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 6], (unsigned long long)total);
+    nc_count32(a.ctr, blk, 7, n_acc);
   }
 }
 

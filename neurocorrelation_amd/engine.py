@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import DriveSpec, Params, SimSpec, WiringSpec, check, lib, ptr
+from ._lib import DriveSpec, LaunchCfg, Params, SimSpec, WiringSpec, check, lib, ptr
 
 NEVER = _lib.NC_NEVER_STEP
 
@@ -194,6 +194,24 @@ class Simulation:
                 "fired_in_degree", "fired_out_degree", "accepted_spikes", "pending_scanned",
                 "mask_words_read", "mask_words_written"]
         return dict(zip(keys, (int(x) for x in out)))
+
+    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0):
+        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps)
+        check(lib().nc_sim_set_launch(self._h, C.byref(cfg)))
+
+    def launch(self):
+        cfg = LaunchCfg()
+        check(lib().nc_sim_get_launch(self._h, C.byref(cfg)))
+        return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
+                "graph_steps": cfg.graph_steps}
+
+    def tune(self, steps_per_trial=32):
+        """Autotunes the launch shape on the live simulation (advances it by
+        2 * 16 * steps_per_trial steps); returns the chosen shape."""
+        cfg = LaunchCfg()
+        check(lib().nc_sim_tune(self._h, steps_per_trial, C.byref(cfg)))
+        return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
+                "graph_steps": cfg.graph_steps}
 
     def set_counting(self, enable):
         check(lib().nc_set_counting(self._h, 1 if enable else 0))

@@ -1,0 +1,112 @@
+"""GPU: the batched form.  Many networks of uneven size and step budget in
+one simulation, under every launch shape (waves per workgroup, XCD-ordered
+blocks, hipGraph-captured step chunks) and after autotuning, give exactly
+the results of the default launch, which the oracle pins elsewhere."""
+import sys
+
+import numpy as np
+import pytest
+
+import neurocorrelation_amd as nc
+from neurocorrelation_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def _mix(seeds=(0, 1, 2, 3, 4, 5), budgets=(300, 900, 650, 1000, 120, 777)):
+    nets = []
+    for s in seeds:
+        net, drive, params, _ = W.build("plain-300", seed=s)
+        drive.bg_prob = 1.0 / 600 if s % 2 else 0.0
+        nets.append((net, drive, params))
+    big = W.build("plain-20k", seed=7, wiring={"n_neurons": 3000, "side": W.side_for(3000)})
+    nets.append((big[0], big[1], big[2]))
+    budgets = list(budgets) + [1000]
+    return nets, budgets
+
+
+def _run(nets, budgets, cfg=None, tune=0, calls=(250, 250, 250, 250)):
+    pos = sum(n.device_size()[0] for n, _, _ in nets)
+    slots = sum(n.device_size()[1] for n, _, _ in nets)
+    syn = sum(n.n_syn for n, _, _ in nets)
+    sim = nc.Simulation(pos, syn, max_nets=len(nets), raster_capacity=1 << 22, max_slots=slots)
+    ids = [sim.add(n, p, d, steps=b) for (n, d, p), b in zip(nets, budgets)]
+    if cfg is not None:
+        sim.set_launch(*cfg)
+    chosen = None
+    if tune:
+        chosen = sim.tune(tune)
+    for k in calls:
+        sim.step(k)
+    out = []
+    for (n, _, _), i in zip(nets, ids):
+        out.append((sim.raster(i, 1 << 22), sim.potential(i, n.n), sim.weights(i, n.n_syn), sim.progress(i)))
+    sim.close()
+    return out, chosen
+
+
+def _same(a, b):
+    for (ra, va, wa, pa), (rb, vb, wb, pb) in zip(a, b):
+        assert pa == pb
+        assert np.array_equal(ra[0], rb[0]) and np.array_equal(ra[1], rb[1])
+        assert np.array_equal(va.view(np.uint32), vb.view(np.uint32))
+        assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+
+
+@pytest.fixture(scope="module")
+def mix_and_ref():
+    nets, budgets = _mix()
+    say("batched reference run: default launch")
+    ref, _ = _run(nets, budgets)
+    return nets, budgets, ref
+
+
+@pytest.mark.parametrize("cfg", [(1, 0, 0), (2, 1, 0), (8, 0, 0), (4, 1, 16), (1, 0, 64), (8, 1, 200)])
+def test_launch_shapes_give_identical_results(mix_and_ref, cfg):
+    nets, budgets, ref = mix_and_ref
+    say("launch shape", cfg)
+    got, _ = _run(nets, budgets, cfg=cfg)
+    _same(got, ref)
+
+
+def test_tuned_launch_gives_identical_results(mix_and_ref):
+    nets, budgets, ref = mix_and_ref
+    say("autotune, 8 steps per trial")
+    got, chosen = _run(nets, budgets, tune=8, calls=(744,))  # 32 trials x 8 steps + 744 = 1000
+    say("chosen", chosen)
+    assert chosen["waves_per_block"] in (1, 2, 4, 8)
+    _same(got, ref)
+
+
+def test_graph_chunks_with_host_events():
+    """Host-scheduled fires force direct launches for the steps that carry
+    them; the captured chunks and the direct steps interleave exactly."""
+    net, drive, params, _ = W.build("plain-300", seed=9)
+    ev_steps = np.array([5, 40, 41, 333, 600], np.uint32)
+    ids = np.array([3, 17, 17, 250, 0], np.uint32)
+    kinds = np.array([0, 1, 0, 1, 0], np.uint8)
+
+    def run(cfg):
+        sim = nc.Simulation.for_network(net, raster_capacity=1 << 20)
+        i = sim.add(net, params, drive)
+        sim.set_launch(*cfg)
+        sim.step(64)
+        sim.push_forced(i, ev_steps, ids, kinds)
+        for _ in range(5):
+            sim.step(160)
+        r = sim.raster(i, 1 << 20)
+        v = sim.potential(i, net.n)
+        w = sim.weights(i, net.n_syn)
+        sim.close()
+        return r, v, w
+
+    a = run((4, 0, 0))
+    b = run((2, 1, 32))
+    assert np.array_equal(a[0][0], b[0][0]) and np.array_equal(a[0][1], b[0][1])
+    assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+    assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
