@@ -143,10 +143,109 @@ def cpu_baseline(net, drive, budget_s=12.0, sample_neurons=40_000):
                       "synapses, %d steps; 1 thread on %s" % (M, S, steps, cpu_model)}
 
 
+def bench_batched(args):
+    """The batched form: `args.nets` independent networks (log-uniform size,
+    uneven degree, per-network seed and input correlation) packed into one
+    simulation, launch shape autotuned on the live run, every network run for
+    its full step budget.  Beside it, a sample of the same networks run one
+    after another on the plain path (one simulation each, default launch),
+    whose final state must equal the batched run's bit for bit."""
+    import concurrent.futures as cf
+
+    import neurocorrelation_amd as nc
+    from neurocorrelation_amd import workloads as W
+
+    steps = args.steps
+    specs = W.batched_specs(count=args.nets, seed=0, steps=steps)
+    t0 = time.perf_counter()
+    with cf.ThreadPoolExecutor(16) as ex:
+        built = list(ex.map(W.build_spec, specs))
+    sizes = [b[0].device_size() for b in built]
+    n_syn = [b[0].n_syn for b in built]
+    n_neu = [b[0].n for b in built]
+    S_tot, N_tot = sum(n_syn), sum(n_neu)
+    print("[bench] batched: %d networks, %d neurons, %d synapses, built in %.1fs" %
+          (len(built), N_tot, S_tot, time.perf_counter() - t0), file=sys.stderr, flush=True)
+    sim = nc.Simulation(sum(p for p, _ in sizes), S_tot, max_nets=len(built), raster_capacity=0,
+                        max_slots=sum(s for _, s in sizes))
+    rng = np.random.default_rng(1)
+    sample = sorted(rng.choice(len(built), size=min(args.sample, len(built)), replace=False).tolist())
+    ids = []
+    for j, (net, drive, params, st) in enumerate(built):
+        ids.append(sim.add(net, params, drive, steps=st))
+        if j not in sample:
+            net.close()
+        if (j + 1) % 1024 == 0:
+            print("[bench] added %d networks" % (j + 1), file=sys.stderr, flush=True)
+    setup_s = time.perf_counter() - t0
+    print("[bench] setup %.1fs" % setup_s, file=sys.stderr, flush=True)
+
+    sim.synchronize()
+    t_start = time.perf_counter()
+    chosen = sim.tune(args.tune_steps)
+    done = 2 * 16 * args.tune_steps
+    print("[bench] tuned launch %s after %d steps (%.2fs)" % (chosen, done, time.perf_counter() - t_start),
+          file=sys.stderr, flush=True)
+    while done < steps:
+        k = min(args.chunk, steps - done)
+        sim.step(k)
+        done += k
+        if done % 5000 < args.chunk:
+            sim.synchronize()
+            print("[bench] batched step %d (%.1fs)" % (done, time.perf_counter() - t_start), file=sys.stderr,
+                  flush=True)
+    sim.synchronize()
+    elapsed = time.perf_counter() - t_start
+    assert all(sim.progress(i)[1] == 0 for i in ids)
+    mine = {j: (sim.potential(ids[j], n_neu[j]), sim.weights(ids[j], n_syn[j]), sim.firings(ids[j], n_neu[j]))
+            for j in sample}
+    sim.close()
+
+    # the same networks one after another on the plain path (a sample)
+    seq_t = []
+    for j in sample:
+        net, drive, params, st = built[j]
+        alone = nc.Simulation.for_network(net)
+        a = alone.add(net, params, drive)
+        alone.synchronize()
+        t = time.perf_counter()
+        left = st
+        while left:
+            k = min(args.chunk, left)
+            alone.step(k)
+            left -= k
+        alone.synchronize()
+        seq_t.append(time.perf_counter() - t)
+        V, w, f = mine[j]
+        assert np.array_equal(alone.potential(a, n_neu[j]).view(np.uint32), V.view(np.uint32)), "V differs, net %d" % j
+        assert np.array_equal(alone.weights(a, n_syn[j]).view(np.uint32), w.view(np.uint32)), "w differs, net %d" % j
+        assert np.array_equal(alone.firings(a, n_neu[j]), f), "firings differ, net %d" % j
+        alone.close()
+        net.close()
+    seq_per_sim = float(np.mean(seq_t))
+    sims_s = len(built) / elapsed
+    out = {
+        "metric": "batched: sims/s and synapse-updates/s vs sequential plain runs",
+        "value": sims_s, "unit": "sims/s", "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": elapsed * 1000.0 / steps, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "batched-%d" % len(built), "networks": len(built), "neurons": N_tot,
+                   "synapses": S_tot, "steps_per_network": steps, "launch": chosen,
+                   "tune_steps_per_trial": args.tune_steps},
+        "synapse_updates_per_s": S_tot * steps / elapsed,
+        "sequential_plain": {"sample": len(sample), "mean_s_per_sim": seq_per_sim, "sims_per_s": 1.0 / seq_per_sim,
+                             "identical_final_state": True},
+        "speedup_vs_sequential": sims_s * seq_per_sim,
+        "setup_s": setup_s,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10_000)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default 10,000 for plain workloads, 20,000 per network for batched-*)")
     ap.add_argument("--warmup", type=int, default=200)
     ap.add_argument("--workload", default="plain-1m")
     ap.add_argument("--chunk", type=int, default=500)
@@ -155,6 +254,9 @@ def main():
                     help="leave HIP events out of the timed region (for external profilers)")
     ap.add_argument("--no-counts", action="store_true",
                     help="skip the counting replay (profiler runs: every launch is then uninstrumented)")
+    ap.add_argument("--nets", type=int, default=4096, help="batched-* workloads: number of networks")
+    ap.add_argument("--sample", type=int, default=12, help="batched-*: networks rerun on the plain path")
+    ap.add_argument("--tune-steps", type=int, default=16, help="batched-*: steps per autotuner trial")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -162,10 +264,16 @@ def main():
         copy_calibration(args.calibrate_copy)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU: every box in this setup has a single MI355X; "
-              "independent networks shard across GPUs without communication, see DESIGN.md",
+        print("bench.py measures one GPU (N = 1): the plain path's single network is measured on one "
+              "MI355X; independent networks would shard across GPUs without communication",
               file=sys.stderr)
         sys.exit(2)
+
+    if args.workload.startswith("batched"):
+        args.steps = args.steps or 20_000
+        bench_batched(args)
+        return
+    args.steps = args.steps or 10_000
 
     import neurocorrelation_amd as nc
     from neurocorrelation_amd import workloads as W

@@ -52,6 +52,45 @@ WORKLOADS = {
 }
 
 
+def batched_specs(count=4096, seed=0, n_range=(200, 20_000), k_range=(10, 60), steps=20_000):
+    """The batched form's networks: neuron counts log-uniform in n_range,
+    K (synapses per neuron in the bulk) uniform in k_range, each wired by the
+    300-neuron workload's rule (density 300 per unit volume, radius for K,
+    delays 1..20 steps proportional to distance, w ~ U[0.2, 0.8)); per-network
+    seed and input-correlation amplitude uniform in [0, 1].  Inputs follow the
+    300-neuron workload's drive scaled with N: 2% of the neurons (at least 6),
+    two thirds of them in 2 sine-modulated groups at p = 0.05 (1 + amp
+    sin(2 pi t / 200)), the rest independent Bernoulli(0.05).  Returns a list
+    of dicts (wiring, drive, steps) for build_spec()."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        k = float(rng.uniform(*k_range))
+        net_seed = int(rng.integers(0, 2**63 - 1))
+        amp = float(rng.uniform(0.0, 1.0))
+        n_in = max(6, int(round(0.02 * n)))
+        n_corr = 2 * ((2 * n_in // 3) // 2)
+        out.append(dict(
+            wiring=dict(n_neurons=n, side=side_for(n), radius=radius_for_degree(k), delay_min=1, delay_max=20,
+                        w_min=0.2, w_max=0.8, seed=net_seed),
+            drive=dict(n_inputs=n_in, n_groups=2, n_correlated=n_corr, period=200, p0=0.05, amp_corr=amp,
+                       amp_ind=0.0, seed=net_seed),
+            steps=steps, k=k))
+    return out
+
+
+def build_spec(spec, n_threads=1):
+    """(Network, DriveSpec, Params, steps) of one batched_specs() entry."""
+    w = dict(spec["wiring"])
+    seed = w.pop("seed")
+    d = dict(spec["drive"])
+    dseed = d.pop("seed")
+    net = Network.generate(seed=seed, spatial_order=1, n_threads=n_threads, **w)
+    return net, drive_spec(d, seed=dseed), default_params(), spec["steps"]
+
+
 def drive_spec(d, seed=0, bg_prob=0.0):
     return DriveSpec(d["n_inputs"], d["n_groups"], d["n_correlated"], d["period"], d["p0"], d["amp_corr"],
                      d["amp_ind"], bg_prob, seed)
