@@ -241,6 +241,67 @@ def bench_batched(args):
     print(json.dumps(out), flush=True)
 
 
+def bench_served(args):
+    """The served form: a Poisson stream of jobs (`args.rate` per second for
+    `args.window` seconds; network size, degree and step count drawn per job)
+    into one running simulation, admitted by free device room and batched
+    continuously; completion latency from arrival (network construction
+    included) and sims/s.  Beside it, a sample of the same jobs run one
+    after another on the plain path (construction included), whose results
+    must equal the served run's bit for bit."""
+    import neurocorrelation_amd as nc
+    from neurocorrelation_amd import serve as SV
+
+    rng = np.random.default_rng(7)
+    n_jobs = int(round(args.rate * args.window))
+    specs = SV.served_specs(n_jobs, seed=3, n_range=(300, args.max_neurons), k_range=(20, 100),
+                            steps_range=(args.min_job_steps, args.max_job_steps))
+    arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=n_jobs))
+    jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
+    probe = nc.Simulation(64, 64)
+    free = probe.free_bytes()
+    probe.close()
+    max_n, max_s = SV.Server.capacity_for(free, fraction=0.5)
+    server = SV.Server(max_n, max_s, max_nets=4096, chunk=args.chunk)
+    print("[bench] served: %d jobs over %.0fs, device room %d neurons / %d synapses" % (n_jobs, args.window, max_n,
+          max_s), file=sys.stderr, flush=True)
+    done, rest = server.run(jobs, t_limit=args.wall_cap, log=lambda m: print("[bench] " + m, file=sys.stderr,
+                                                                                  flush=True))
+    server.close()
+    lat = np.array([j.t_done - j.arrival for j in done])
+    t_end = max(j.t_done for j in done) if done else 0.0
+    syn_steps = sum(j.n_syn * j.steps for j in done)
+    # the same jobs alone, one after another, on the plain path (a sample of the completed ones)
+    sample = sorted(rng.choice(len(done), size=min(args.sample, len(done)), replace=False).tolist()) if done else []
+    seq_t = []
+    for i in sample:
+        job = done[i]
+        res, el = SV.run_alone(job.spec)
+        assert res == job.result, "job %d differs alone and in the mix" % job.idx
+        seq_t.append(el)
+    seq_per_job = float(np.mean(seq_t)) if seq_t else None
+    out = {
+        "metric": "served: sims/s and p50/p99 completion latency vs sequential plain runs",
+        "value": len(done) / t_end if t_end else None, "unit": "sims/s", "n_gpus": 1, "steps": None,
+        "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32", "data": "synthetic",
+        "config": {"workload": "served", "rate_per_s": args.rate, "window_s": args.window, "jobs": n_jobs,
+                   "neurons": [300, args.max_neurons], "synapses_per_neuron": [20, 100],
+                   "job_steps": [args.min_job_steps, args.max_job_steps], "chunk": args.chunk,
+                   "wall_cap_s": args.wall_cap},
+        "completed": len(done), "unfinished": rest["unfinished"],
+        "latency_s": {"p50": float(np.percentile(lat, 50)) if len(lat) else None,
+                      "p99": float(np.percentile(lat, 99)) if len(lat) else None,
+                      "max": float(lat.max()) if len(lat) else None},
+        "synapse_updates_per_s": syn_steps / t_end if t_end else None,
+        "sequential_plain": {"sample": len(sample), "mean_s_per_job": seq_per_job,
+                             "sims_per_s": 1.0 / seq_per_job if seq_per_job else None,
+                             "identical_results": True},
+        "speedup_vs_sequential": (len(done) / t_end) * seq_per_job if (t_end and seq_per_job) else None,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +318,12 @@ def main():
     ap.add_argument("--nets", type=int, default=4096, help="batched-* workloads: number of networks")
     ap.add_argument("--sample", type=int, default=12, help="batched-*: networks rerun on the plain path")
     ap.add_argument("--tune-steps", type=int, default=16, help="batched-*: steps per autotuner trial")
+    ap.add_argument("--rate", type=float, default=200.0, help="served: job arrivals per second")
+    ap.add_argument("--window", type=float, default=60.0, help="served: seconds of arrivals")
+    ap.add_argument("--max-neurons", type=int, default=100_000, help="served: largest job network")
+    ap.add_argument("--min-job-steps", type=int, default=1_000, help="served: shortest job")
+    ap.add_argument("--max-job-steps", type=int, default=50_000, help="served: longest job")
+    ap.add_argument("--wall-cap", type=float, default=900.0, help="served: stop serving after this many seconds")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -272,6 +339,9 @@ def main():
     if args.workload.startswith("batched"):
         args.steps = args.steps or 20_000
         bench_batched(args)
+        return
+    if args.workload == "served":
+        bench_served(args)
         return
     args.steps = args.steps or 10_000
 

@@ -1,0 +1,187 @@
+"""The served form: a request loop over one device-resident simulation.
+
+Jobs (a network spec, a drive, a step count) arrive over time.  Each is
+built on a host thread as it arrives, admitted into the running simulation
+as soon as the simulation's device pools (sized from free HBM) have room for
+it, stepped together with every other admitted job (continuous batching: one
+step kernel pair covers all of them, each job stops at its own budget), and
+retired when its budget is spent, which frees its room for the jobs waiting.
+A job's result is exactly what it gives when run alone: the step of one
+network does not depend on the others in the simulation.
+"""
+import collections
+import concurrent.futures as cf
+import hashlib
+import math
+import time
+
+import numpy as np
+
+from . import workloads
+from .engine import Simulation, footprint
+from ._lib import NCError
+
+
+def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps_range=(1_000, 50_000)):
+    """Jobs of the served form: neuron count log-uniform in n_range, K
+    (synapses per neuron in the bulk) uniform in k_range, step count uniform
+    in steps_range; wiring and drive of the 20,000-neuron workload (density
+    300 per unit volume, delays 1..32 steps, w ~ U[0.2, 0.8), 5% inputs in 4
+    correlated groups plus independent ones, p = 0.05 (1 + sin(2 pi t /
+    200))), seeded per job."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(n_range[0]), math.log(n_range[1])))))
+        k = float(rng.uniform(*k_range))
+        steps = int(rng.integers(steps_range[0], steps_range[1] + 1))
+        net_seed = int(rng.integers(0, 2**63 - 1))
+        n_in = max(5, int(round(0.05 * n)))
+        n_corr = 4 * ((4 * n_in // 5) // 4)
+        out.append(dict(
+            wiring=dict(n_neurons=n, side=workloads.side_for(n), radius=workloads.radius_for_degree(k),
+                        delay_min=1, delay_max=32, w_min=0.2, w_max=0.8, seed=net_seed),
+            drive=dict(n_inputs=n_in, n_groups=4, n_correlated=n_corr, period=200, p0=0.05, amp_corr=1.0,
+                       amp_ind=1.0, seed=net_seed),
+            steps=steps, k=k))
+    return out
+
+
+def digest(sim, nid, n, n_syn):
+    """A job's result: its firing counts and a hash of its final potentials
+    and weights (bit patterns)."""
+    V = sim.potential(nid, n)
+    w = sim.weights(nid, n_syn)
+    f = sim.firings(nid, n)
+    h = hashlib.sha256()
+    h.update(V.tobytes())
+    h.update(w.tobytes())
+    h.update(f.tobytes())
+    return {"spikes": int(f.sum()), "hash": h.hexdigest()}
+
+
+class Job:
+    __slots__ = ("idx", "spec", "arrival", "built", "net", "drive", "params", "steps", "n", "n_syn", "nid",
+                 "t_built", "t_admit", "t_done", "result")
+
+    def __init__(self, idx, spec, arrival):
+        self.idx, self.spec, self.arrival = idx, spec, arrival
+        self.net = self.nid = self.t_built = self.t_admit = self.t_done = self.result = None
+
+
+def _build(job):
+    net, drive, params, steps = workloads.build_spec(job.spec, n_threads=1)
+    job.net, job.drive, job.params, job.steps = net, drive, params, steps
+    job.n, job.n_syn = net.n, net.n_syn
+    return job
+
+
+class Server:
+    """One simulation holding up to `max_nets` jobs within `max_neurons` /
+    `max_synapses` of device room; `chunk` steps between admission and
+    retirement checks."""
+
+    def __init__(self, max_neurons, max_synapses, max_nets=4096, chunk=200, builders=8, device=0):
+        slots = max_synapses + max_synapses // 4 + (max_neurons + 64 * max_nets) * 4
+        self.sim = Simulation(max_neurons, max_synapses, max_nets=max_nets, device=device, max_slots=slots)
+        self.chunk = chunk
+        self.pool = cf.ThreadPoolExecutor(builders)
+
+    @staticmethod
+    def capacity_for(free_bytes, fraction=0.5, syn_per_neuron=60):
+        """(max_neurons, max_synapses) whose footprint is about `fraction`
+        of `free_bytes`, for networks of about `syn_per_neuron` synapses."""
+        per_neuron = footprint(1_000_000, 1_000_000 * syn_per_neuron) / 1_000_000
+        n = int(free_bytes * fraction / per_neuron)
+        return n, n * syn_per_neuron
+
+    def run(self, jobs, t_limit=None, log=None):
+        """Serves `jobs` (each with an arrival time in seconds from now).
+        Returns the jobs with build/admit/done times and results."""
+        t0 = time.perf_counter()
+        arrivals = sorted(jobs, key=lambda j: j.arrival)
+        nxt = 0
+        building = set()
+        waiting = collections.deque()
+        running = {}
+        done = []
+        last_log = t0
+        while nxt < len(arrivals) or building or waiting or running:
+            now = time.perf_counter() - t0
+            if t_limit is not None and now > t_limit:
+                break
+            while nxt < len(arrivals) and arrivals[nxt].arrival <= now:
+                building.add(self.pool.submit(_build, arrivals[nxt]))
+                nxt += 1
+            for fut in [f for f in building if f.done()]:
+                building.discard(fut)
+                job = fut.result()
+                job.t_built = time.perf_counter() - t0
+                waiting.append(job)
+            # admission in arrival order while the device pools have room
+            while waiting:
+                job = waiting[0]
+                try:
+                    job.nid = self.sim.add(job.net, job.params, job.drive, steps=job.steps)
+                except NCError as e:
+                    if e.code != 5:  # anything but NC_ERR_CAPACITY
+                        raise
+                    if running:  # wait for room
+                        break
+                    # nothing runs and it still does not fit: it never will
+                    waiting.popleft()
+                    job.result = {"error": "larger than the simulation's device room"}
+                    job.t_done = time.perf_counter() - t0
+                    job.net.close()
+                    job.net = None
+                    done.append(job)
+                    continue
+                waiting.popleft()
+                job.t_admit = time.perf_counter() - t0
+                running[job.nid] = job
+            if running:
+                self.sim.step(self.chunk)
+                self.sim.synchronize()
+                for nid in list(running):
+                    if self.sim.progress(nid)[1] == 0:
+                        job = running.pop(nid)
+                        job.result = digest(self.sim, nid, job.n, job.n_syn)
+                        job.t_done = time.perf_counter() - t0
+                        self.sim.remove(nid)
+                        job.net.close()
+                        job.net = None
+                        done.append(job)
+            else:
+                wait = 0.001
+                if nxt < len(arrivals):
+                    wait = max(0.0, min(0.01, arrivals[nxt].arrival - (time.perf_counter() - t0)))
+                time.sleep(wait)
+            if log and time.perf_counter() - last_log > 10.0:
+                last_log = time.perf_counter()
+                log("served %d / %d jobs, %d running, %d waiting, %.1fs" %
+                    (len(done), len(arrivals), len(running), len(waiting), last_log - t0))
+        return done, {"unfinished": len(arrivals) - len(done), "running": len(running), "waiting": len(waiting)}
+
+    def close(self):
+        self.pool.shutdown(wait=True)
+        self.sim.close()
+
+
+def run_alone(spec):
+    """One job on the plain path: its own simulation, default launch.
+    Returns (result, seconds including construction)."""
+    t = time.perf_counter()
+    net, drive, params, steps = workloads.build_spec(spec, n_threads=0)
+    sim = Simulation.for_network(net)
+    nid = sim.add(net, params, drive)
+    left = steps
+    while left:
+        k = min(500, left)
+        sim.step(k)
+        left -= k
+    sim.synchronize()
+    res = digest(sim, nid, net.n, net.n_syn)
+    el = time.perf_counter() - t
+    sim.close()
+    net.close()
+    return res, el

@@ -1,0 +1,47 @@
+"""GPU: the served form.  Jobs arriving over time into one running
+simulation, admitted as room frees up and batched with whatever else is
+running, give exactly the result each gives alone on the plain path."""
+import sys
+
+import numpy as np
+import pytest
+
+from neurocorrelation_amd import serve as SV
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_served_jobs_equal_alone_under_queueing():
+    specs = SV.served_specs(10, seed=11, n_range=(300, 2500), k_range=(20, 40), steps_range=(150, 700))
+    rng = np.random.default_rng(5)
+    arrivals = np.cumsum(rng.exponential(0.02, size=len(specs)))
+    jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
+    # room for a few jobs at a time: later arrivals wait for earlier ones to finish
+    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=4, chunk=64, builders=4)
+    say("serving", len(jobs), "jobs")
+    done, rest = server.run(jobs, t_limit=120)
+    server.close()
+    assert rest["unfinished"] == 0 and len(done) == len(jobs)
+    assert any(j.t_admit > j.t_built + 1e-3 for j in done)  # someone waited for room
+    for job in sorted(done, key=lambda j: j.idx):
+        say("job", job.idx, "n", job.n, "steps", job.steps)
+        res, _ = SV.run_alone(job.spec)
+        assert res == job.result, "job %d differs alone and in the mix" % job.idx
+
+
+def test_job_larger_than_the_room_is_refused():
+    specs = SV.served_specs(2, seed=12, n_range=(400, 400), k_range=(20, 20), steps_range=(50, 50))
+    big = SV.served_specs(1, seed=13, n_range=(20_000, 20_000), k_range=(30, 30), steps_range=(50, 50))
+    jobs = [SV.Job(0, specs[0], 0.0), SV.Job(1, big[0], 0.0), SV.Job(2, specs[1], 0.01)]
+    server = SV.Server(max_neurons=4000, max_synapses=60_000, max_nets=4, chunk=32, builders=2)
+    done, rest = server.run(jobs, t_limit=60)
+    server.close()
+    by = {j.idx: j for j in done}
+    assert rest["unfinished"] == 0
+    assert "error" in by[1].result
+    assert "hash" in by[0].result and "hash" in by[2].result
