@@ -249,12 +249,15 @@ nc_status nc_set_counting(nc_sim* sim, int enable);
  * nc_sim.  waves_per_block: 1, 2, 4 or 8 slices (64 neurons each) per
  * workgroup; xcd_swizzle: each of the 8 XCDs takes one contiguous range of
  * slices; graph_steps: steps per captured hipGraph chunk (0 = direct
- * launches; a chunk is replayed while the set of networks is unchanged).
- * Results do not depend on the launch shape.  Default {4, 0, 0}. */
+ * launches; a chunk is replayed while the set of networks is unchanged);
+ * stage_tables: copy the STDP trace tables into LDS per workgroup (pays when
+ * many synapses see STDP per step) instead of reading them through the
+ * caches.  Results do not depend on the launch shape.  Default {4, 0, 0, 0}. */
 typedef struct nc_launch_cfg {
   uint32_t waves_per_block;
   uint32_t xcd_swizzle;
   uint32_t graph_steps;
+  uint32_t stage_tables;
 } nc_launch_cfg;
 nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg);
 nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg);

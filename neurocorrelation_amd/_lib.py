@@ -49,7 +49,8 @@ class DriveSpec(C.Structure):
 
 
 class LaunchCfg(C.Structure):
-    _fields_ = [("waves_per_block", C.c_uint32), ("xcd_swizzle", C.c_uint32), ("graph_steps", C.c_uint32)]
+    _fields_ = [("waves_per_block", C.c_uint32), ("xcd_swizzle", C.c_uint32), ("graph_steps", C.c_uint32),
+                ("stage_tables", C.c_uint32)]
 
 
 class SimSpec(C.Structure):

@@ -261,7 +261,7 @@ struct nc_sim {
   unsigned long long* ctr = nullptr;
   bool counting = false;
   uint32_t* d_chunk_off = nullptr;  // steps of the current call before the running chunk
-  nc_launch_cfg cfg{4, 0, 0};
+  nc_launch_cfg cfg{4, 0, 0, 0};
   // captured chunk of cfg.graph_steps steps and the launch it was captured for
   hipGraphExec_t graph = nullptr;
   nc_step_args graph_args{};
@@ -346,7 +346,7 @@ nc_step_args base_args(nc_sim* sim) {
   a.w_ap = sim->w_ap;
   a.len_pre = (int32_t)sim->T.t_pre.size();
   a.len_post = (int32_t)sim->T.t_post.size();
-  a.tabs_in_lds = (uint64_t)(a.len_pre + a.len_post) * 4 <= NC_LDS_TAB_BYTES ? 1 : 0;
+  a.tabs_in_lds = sim->cfg.stage_tables && (uint64_t)(a.len_pre + a.len_post) * 4 <= NC_LDS_TAB_BYTES ? 1 : 0;
   a.chunk_off = sim->d_chunk_off;
   a.xcd_swizzle = sim->cfg.xcd_swizzle;
   for (int k = 0; k < NC_WAP_K; ++k)
@@ -1276,6 +1276,7 @@ extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) {
   if (w != 1 && w != 2 && w != 4 && w != 8) return fail(NC_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
   if (cfg->xcd_swizzle > 1) return fail(NC_ERR_ARG, "xcd_swizzle must be 0 or 1");
   if (cfg->graph_steps > 4096) return fail(NC_ERR_ARG, "graph_steps must be at most 4096");
+  if (cfg->stage_tables > 1) return fail(NC_ERR_ARG, "stage_tables must be 0 or 1");
   sim->cfg = *cfg;
   return NC_OK;
 }
@@ -1303,8 +1304,8 @@ extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launc
   const uint32_t graphs[2] = {0, steps_per_trial};
   for (uint32_t g : graphs)
     for (uint32_t w : waves)
-      for (uint32_t x = 0; x < 2 && st == NC_OK; ++x) {
-        sim->cfg = nc_launch_cfg{w, x, g};
+      for (uint32_t x = 0; x < 4 && st == NC_OK; ++x) {
+        sim->cfg = nc_launch_cfg{w, x & 1u, g, x >> 1};
         st = nc_sim_step(sim, steps_per_trial);  // warm-up (and graph capture)
         if (st != NC_OK) break;
         if (hipEventRecord(e0, sim->stream) != hipSuccess) {

@@ -383,10 +383,13 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   if (wave_in && lane == 0)
     a.fired[wave] = live ? make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q) : make_uint4(0u, 0u, 0u, 0u);
   if (fm) {
-    // The row passes below read slot records this wave wrote above (arrival
-    // STDP), from other lanes: make those writes visible first.
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    // The row passes below read the fired lanes' slot records from other
+    // lanes; when a fired lane wrote some of them above (arrival STDP), make
+    // those writes visible first.
+    if (__ballot(fire && n_arr != 0)) {
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+    }
     // Post-side STDP of each fired lane (the inSynapses loop of Neuron::fire,
     // src/NeuCor.cpp:651-653): target trace powf(decay, 0) == 1 -> 0, synapse
     // trace of lastSpikeArrival from before this step's arrivals.  The whole

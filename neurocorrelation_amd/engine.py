@@ -195,23 +195,23 @@ class Simulation:
                 "mask_words_read", "mask_words_written"]
         return dict(zip(keys, (int(x) for x in out)))
 
-    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0):
-        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps)
+    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0, stage_tables=0):
+        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps, stage_tables)
         check(lib().nc_sim_set_launch(self._h, C.byref(cfg)))
 
     def launch(self):
         cfg = LaunchCfg()
         check(lib().nc_sim_get_launch(self._h, C.byref(cfg)))
         return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
-                "graph_steps": cfg.graph_steps}
+                "graph_steps": cfg.graph_steps, "stage_tables": cfg.stage_tables}
 
     def tune(self, steps_per_trial=32):
         """Autotunes the launch shape on the live simulation (advances it by
-        2 * 16 * steps_per_trial steps); returns the chosen shape."""
+        2 * 32 * steps_per_trial steps); returns the chosen shape."""
         cfg = LaunchCfg()
         check(lib().nc_sim_tune(self._h, steps_per_trial, C.byref(cfg)))
         return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
-                "graph_steps": cfg.graph_steps}
+                "graph_steps": cfg.graph_steps, "stage_tables": cfg.stage_tables}
 
     def set_counting(self, enable):
         check(lib().nc_set_counting(self._h, 1 if enable else 0))

@@ -19,10 +19,10 @@ def say(*a):
 def test_served_jobs_equal_alone_under_queueing():
     specs = SV.served_specs(10, seed=11, n_range=(300, 2500), k_range=(20, 40), steps_range=(150, 700))
     rng = np.random.default_rng(5)
-    arrivals = np.cumsum(rng.exponential(0.02, size=len(specs)))
+    arrivals = np.concatenate([[0.0] * 4, np.cumsum(rng.exponential(0.02, size=len(specs) - 4))])
     jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
-    # room for a few jobs at a time: later arrivals wait for earlier ones to finish
-    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=4, chunk=64, builders=4)
+    # room for two jobs at a time: four arrive together, so some wait for room
+    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=2, chunk=64, builders=4)
     say("serving", len(jobs), "jobs")
     done, rest = server.run(jobs, t_limit=120)
     server.close()
