@@ -254,8 +254,9 @@ def bench_served(args):
 
     rng = np.random.default_rng(7)
     n_jobs = int(round(args.rate * args.window))
+    weights = (0.2, 0.8, 0.0) if args.runaway else (0.05, 0.3, 0.2)
     specs = SV.served_specs(n_jobs, seed=3, n_range=(300, args.max_neurons), k_range=(20, 100),
-                            steps_range=(args.min_job_steps, args.max_job_steps))
+                            steps_range=(args.min_job_steps, args.max_job_steps), weights=weights)
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=n_jobs))
     jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
     probe = nc.Simulation(64, 64)
@@ -288,6 +289,7 @@ def bench_served(args):
         "config": {"workload": "served", "rate_per_s": args.rate, "window_s": args.window, "jobs": n_jobs,
                    "neurons": [300, args.max_neurons], "synapses_per_neuron": [20, 100],
                    "job_steps": [args.min_job_steps, args.max_job_steps], "chunk": args.chunk,
+                   "weights": {"w_min": weights[0], "w_max": weights[1], "inhib_frac": weights[2]},
                    "wall_cap_s": args.wall_cap},
         "completed": len(done), "unfinished": rest["unfinished"],
         "latency_s": {"p50": float(np.percentile(lat, 50)) if len(lat) else None,
@@ -324,6 +326,8 @@ def main():
     ap.add_argument("--min-job-steps", type=int, default=1_000, help="served: shortest job")
     ap.add_argument("--max-job-steps", type=int, default=50_000, help="served: longest job")
     ap.add_argument("--wall-cap", type=float, default=900.0, help="served: stop serving after this many seconds")
+    ap.add_argument("--runaway", action="store_true",
+                    help="served: all-excitatory weights in [0.2, 0.8) (jobs run at p ~ 0.27 instead of ~0.0025)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a streaming copy of MIB MiB (counter calibration) and exit")
     args = ap.parse_args()

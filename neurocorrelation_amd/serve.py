@@ -22,13 +22,16 @@ from .engine import Simulation, footprint
 from ._lib import NCError
 
 
-def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps_range=(1_000, 50_000)):
+def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps_range=(1_000, 50_000),
+                 weights=(0.05, 0.3, 0.2)):
     """Jobs of the served form: neuron count log-uniform in n_range, K
     (synapses per neuron in the bulk) uniform in k_range, step count uniform
-    in steps_range; wiring and drive of the 20,000-neuron workload (density
-    300 per unit volume, delays 1..32 steps, w ~ U[0.2, 0.8), 5% inputs in 4
-    correlated groups plus independent ones, p = 0.05 (1 + sin(2 pi t /
-    200))), seeded per job."""
+    in steps_range; wiring (density 300 per unit volume, delays 1..32 steps)
+    and drive (5% inputs in 4 correlated groups plus independent ones, p =
+    0.05 (1 + sin(2 pi t / 200))) of the 20,000-neuron workload, seeded per
+    job.  weights = (w_min, w_max, inhibitory fraction): the default is the
+    10^6-neuron workload's rule, under which the jobs run at p ~ 0.0025 per
+    step; (0.2, 0.8, 0.0) makes them run away to p ~ 0.27."""
     rng = np.random.default_rng(seed)
     out = []
     for _ in range(count):
@@ -40,7 +43,8 @@ def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps
         n_corr = 4 * ((4 * n_in // 5) // 4)
         out.append(dict(
             wiring=dict(n_neurons=n, side=workloads.side_for(n), radius=workloads.radius_for_degree(k),
-                        delay_min=1, delay_max=32, w_min=0.2, w_max=0.8, seed=net_seed),
+                        delay_min=1, delay_max=32, w_min=weights[0], w_max=weights[1], inhib_frac=weights[2],
+                        seed=net_seed),
             drive=dict(n_inputs=n_in, n_groups=4, n_correlated=n_corr, period=200, p0=0.05, amp_corr=1.0,
                        amp_ind=1.0, seed=net_seed),
             steps=steps, k=k))
@@ -81,10 +85,11 @@ class Server:
     `max_synapses` of device room; `chunk` steps between admission and
     retirement checks."""
 
-    def __init__(self, max_neurons, max_synapses, max_nets=4096, chunk=200, builders=8, device=0):
+    def __init__(self, max_neurons, max_synapses, max_nets=4096, chunk=200, builders=8, device=0, max_ready=256):
         slots = max_synapses + max_synapses // 4 + (max_neurons + 64 * max_nets) * 4
         self.sim = Simulation(max_neurons, max_synapses, max_nets=max_nets, device=device, max_slots=slots)
         self.chunk = chunk
+        self.max_ready = max_ready  # jobs built or being built but not admitted (bounds host memory)
         self.pool = cf.ThreadPoolExecutor(builders)
 
     @staticmethod
@@ -110,7 +115,8 @@ class Server:
             now = time.perf_counter() - t0
             if t_limit is not None and now > t_limit:
                 break
-            while nxt < len(arrivals) and arrivals[nxt].arrival <= now:
+            while (nxt < len(arrivals) and arrivals[nxt].arrival <= now
+                   and len(building) + len(waiting) < self.max_ready):
                 building.add(self.pool.submit(_build, arrivals[nxt]))
                 nxt += 1
             for fut in [f for f in building if f.done()]:
@@ -153,13 +159,14 @@ class Server:
                         done.append(job)
             else:
                 wait = 0.001
-                if nxt < len(arrivals):
+                if nxt < len(arrivals) and not building:
                     wait = max(0.0, min(0.01, arrivals[nxt].arrival - (time.perf_counter() - t0)))
                 time.sleep(wait)
             if log and time.perf_counter() - last_log > 10.0:
                 last_log = time.perf_counter()
-                log("served %d / %d jobs, %d running, %d waiting, %.1fs" %
-                    (len(done), len(arrivals), len(running), len(waiting), last_log - t0))
+                log("served %d / %d jobs, %d running, %d waiting, %d arrived, %.1fs" %
+                    (len(done), len(arrivals), len(running), len(waiting),
+                     sum(1 for j in arrivals if j.arrival <= last_log - t0), last_log - t0))
         return done, {"unfinished": len(arrivals) - len(done), "running": len(running), "waiting": len(waiting)}
 
     def close(self):
