@@ -212,6 +212,8 @@ nc_status nc_read_firings(nc_sim* sim, int32_t net_id, uint32_t* out, uint32_t n
 /* Synapse::getWeight / setWeight (src/NeuCor.cpp:572-579). */
 nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, uint64_t n);
 nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* w, uint64_t n);
+/* Synapse::setWeight on the listed synapses only (indices in export order). */
+nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, const uint64_t* syn, const float* w);
 
 /* Synapse spike state for the renderer's getPrePot/getPostPot
  * (src/NeuCor.cpp:551-567): arrival step of the spike in flight (0 = none,
@@ -239,6 +241,19 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * Writes the first n (at most NC_NUM_COUNTERS) into out. */
 #define NC_NUM_COUNTERS 11
 nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
+
+/* Checkpoint of one network's dynamic state.  No counterpart in the
+ * reference, which keeps nothing resumable: for the served form's eviction
+ * and resume, and for reproducible fixtures.  nc_sim_save writes potentials,
+ * last fires, firing counts, every synapse's weight, latched depolarisation,
+ * last arrival and arrival step, queued host events, the step count and the
+ * plasticity parameters, in network-local order (independent of where the
+ * network sits in the simulation), into buf; *size receives the bytes needed
+ * (buf = NULL asks for the size only).  nc_sim_restore loads such a state
+ * into a network just added (not yet stepped) from the same wiring at the
+ * same dt, which then continues exactly as the saved one would have. */
+nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_t cap, uint64_t* size);
+nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf, uint64_t size);
 
 /* Device event counters on (enable != 0) or off (the default: the step
  * kernels then carry no instrumentation). */

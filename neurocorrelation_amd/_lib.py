@@ -89,11 +89,14 @@ SIGNATURES = {
     "nc_read_firings": (C.c_int, [P, C.c_int32, u32p, C.c_uint32]),
     "nc_read_weights": (C.c_int, [P, C.c_int32, f32p, C.c_uint64]),
     "nc_write_weights": (C.c_int, [P, C.c_int32, f32p, C.c_uint64]),
+    "nc_set_weights": (C.c_int, [P, C.c_int32, C.c_uint64, u64p, f32p]),
     "nc_read_synapse_state": (C.c_int, [P, C.c_int32, i32p, i32p, f32p, C.c_uint64]),
     "nc_read_raster": (C.c_int, [P, C.c_int32, u32p, u32p, C.c_uint64, u64p]),
     "nc_reset_activities": (C.c_int, [P, C.c_int32]),
     "nc_read_counters": (C.c_int, [P, u64p, C.c_uint32]),
     "nc_set_counting": (C.c_int, [P, C.c_int]),
+    "nc_sim_save": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_uint64, u64p]),
+    "nc_sim_restore": (C.c_int, [P, C.c_int32, C.c_void_p, C.c_uint64]),
     "nc_sim_set_launch": (C.c_int, [P, C.POINTER(LaunchCfg)]),
     "nc_sim_get_launch": (C.c_int, [P, C.POINTER(LaunchCfg)]),
     "nc_sim_tune": (C.c_int, [P, C.c_uint32, C.POINTER(LaunchCfg)]),

@@ -204,6 +204,9 @@ struct NetRec {
   uint64_t mwords_step = 0;  // active-mask words k_neuron reads per step
   std::vector<uint32_t> pos_of_id;    // global positions
   std::vector<uint32_t> slot_of_syn;  // global slot per synapse, export order
+  std::vector<uint32_t> edge_of_syn;  // global out-edge per synapse, export order
+  std::vector<int64_t> in_ptr;        // incoming rows of the network (export order)
+  uint64_t net_digest = 0;            // identity of the network's wiring
   std::vector<uint32_t> id_of_pos;    // local, nq*64
   float lr = 1.f, pre_f = 0.13f, post_f = 0.30f;
   uint32_t steps_done = 0, steps_left = UINT32_MAX;
@@ -657,6 +660,21 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   std::vector<nc_slot> hs(L.nslots);
   for (auto& x : hs) x = nc_slot{0.0f, 0.0f, NC_NEVER, 2 * NC_NEVER};
   r.slot_of_syn.resize(S);
+  r.edge_of_syn.assign(S, 0u);
+  r.in_ptr = net->in_ptr;
+  {
+    // FNV-1a over the wiring: the identity a checkpoint is checked against
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&h](const void* p, size_t n) {
+      const unsigned char* c = (const unsigned char*)p;
+      for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
+    };
+    mix(net->in_ptr.data(), net->in_ptr.size() * 8);
+    mix(net->in_src.data(), net->in_src.size() * 4);
+    mix(net->delay.data(), net->delay.size());
+    mix(net->inh.data(), net->inh.size());
+    r.net_digest = h;
+  }
   for (uint32_t id_ = 0; id_ < net->n; ++id_) {
     const uint32_t pl = L.pos_of_id[id_];
     const uint32_t q = pl / 64, l = pl % 64;
@@ -688,6 +706,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
         uint32_t e = fillp[sp]++ - (uint32_t)o0;
         const uint32_t tq = (uint32_t)q0 + L.pos_of_id[tg] / 64;
         oedge[e] = nc_edge{r.slot_of_syn[k], (tq << 8) | net->delay[k], NC_NEVER, slice_off[tq - q0]};
+        r.edge_of_syn[k] = (uint32_t)o0 + e;
       }
     }
   }
@@ -1178,6 +1197,27 @@ extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* 
   return NC_OK;
 }
 
+extern "C" nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, const uint64_t* syn, const float* w) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (n && (!syn || !w)) return fail(NC_ERR_ARG, "null argument");
+  for (uint64_t i = 0; i < n; ++i)
+    if (syn[i] >= r->S) return fail(NC_ERR_RANGE, "synapse index out of range");
+  std::vector<nc_slot> hs;
+  st = read_slots(sim, *r, &hs);
+  if (st != NC_OK) return st;
+  for (uint64_t i = 0; i < n; ++i) {
+    nc_slot& x = hs[r->slot_of_syn[syn[i]] - r->slot0];
+    x.w = w[i];
+    x.arr_inh = (x.arr_inh & ~1) | ((w[i] < 0.0f) ? 1 : 0);
+  }
+  NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice,
+                        sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return NC_OK;
+}
+
 extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t* arrival,
                                            int32_t* last_arrival, float* depol, uint64_t n) {
   NetRec* r;
@@ -1237,6 +1277,172 @@ extern "C" nc_status nc_read_raster(nc_sim* sim, int32_t net_id, uint32_t* steps
   uint32_t nrest = (uint32_t)rest.size();
   if (nrest) NC_HIP(hipMemcpy(sim->raster, back.data(), back.size() * 4, hipMemcpyHostToDevice));
   NC_HIP(hipMemcpy(sim->raster_cnt, &nrest, 4, hipMemcpyHostToDevice));
+  return NC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Checkpoints: a network's dynamic state in network-local order.
+//   header  : magic 'NCK1', version 1, n, S, steps_done, lr, pre_f, post_f,
+//             dt, wiring digest, queued host events
+//   neurons : V f32, lastFire i32 (step), firings u32          (id order)
+//   synapses: w f32, depol f32, lastArrival i32, arrival i32   (export order)
+//   events  : step offset u32, id u32, kind u32                (queued forced fires)
+// Active masks, their summaries, the pending lists and the fan-out latches
+// are functions of (arrival, steps_done) and are rebuilt on restore.
+namespace {
+struct nc_ckpt_header {
+  uint32_t magic, version, n, pad0;
+  uint64_t n_syn;
+  uint32_t steps_done;
+  float lr, pre_f, post_f, dt;
+  uint32_t n_events;
+  uint64_t digest;
+};
+}  // namespace
+
+extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_t cap, uint64_t* size) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!size) return fail(NC_ERR_ARG, "null size");
+  const uint64_t need = sizeof(nc_ckpt_header) + 12ull * r->n + 16ull * r->S + 12ull * r->events.size();
+  *size = need;
+  if (!buf) return NC_OK;
+  if (cap < need) return fail(NC_ERR_RANGE, "checkpoint buffer too small");
+  std::vector<nc_slot> hs;
+  st = read_slots(sim, *r, &hs);  // applies the deferred arrival STDP first
+  if (st != NC_OK) return st;
+  unsigned char* p = (unsigned char*)buf;
+  nc_ckpt_header h{};
+  h.magic = 0x314b434eu;  // "NCK1"
+  h.version = 1;
+  h.n = r->n;
+  h.n_syn = r->S;
+  h.steps_done = r->steps_done;
+  h.lr = r->lr, h.pre_f = r->pre_f, h.post_f = r->post_f, h.dt = sim->T.dt;
+  h.n_events = (uint32_t)r->events.size();
+  h.digest = r->net_digest;
+  memcpy(p, &h, sizeof h);
+  p += sizeof h;
+  std::vector<float> V(r->n);
+  std::vector<int32_t> lf(r->n);
+  std::vector<uint32_t> fi(r->n);
+  st = read_pos_array(sim, *r, sim->V, V.data(), r->n);
+  if (st == NC_OK) st = read_pos_array(sim, *r, sim->last_fire, lf.data(), r->n);
+  if (st == NC_OK) st = read_pos_array(sim, *r, sim->firings, fi.data(), r->n);
+  if (st != NC_OK) return st;
+  for (uint32_t i = 0; i < r->n; ++i) {
+    memcpy(p, &V[i], 4), memcpy(p + 4, &lf[i], 4), memcpy(p + 8, &fi[i], 4);
+    p += 12;
+  }
+  for (uint64_t k = 0; k < r->S; ++k) {
+    const nc_slot& x = hs[r->slot_of_syn[k] - r->slot0];
+    const int32_t arr = x.arr_inh >> 1;
+    memcpy(p, &x.w, 4), memcpy(p + 4, &x.depol, 4), memcpy(p + 8, &x.last_arr, 4), memcpy(p + 12, &arr, 4);
+    p += 16;
+  }
+  for (auto& ev : r->events) {
+    const uint32_t off = ev.first - r->steps_done, pos = ev.second.first - r->q0 * 64;
+    const uint32_t id = r->id_of_pos[pos], kind = ev.second.second;
+    memcpy(p, &off, 4), memcpy(p + 4, &id, 4), memcpy(p + 8, &kind, 4);
+    p += 12;
+  }
+  return NC_OK;
+}
+
+extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf, uint64_t size) {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!buf || size < sizeof(nc_ckpt_header)) return fail(NC_ERR_ARG, "not a checkpoint");
+  nc_ckpt_header h;
+  memcpy(&h, buf, sizeof h);
+  if (h.magic != 0x314b434eu || h.version != 1) return fail(NC_ERR_ARG, "not a checkpoint (magic/version)");
+  if (h.n != r->n || h.n_syn != r->S || h.digest != r->net_digest)
+    return fail(NC_ERR_ARG, "checkpoint of a different network");
+  if (h.dt != sim->T.dt) return fail(NC_ERR_ARG, "checkpoint taken at another dt");
+  if (r->steps_done != 0) return fail(NC_ERR_STATE, "restore into a network that has not stepped yet");
+  if (size != sizeof(nc_ckpt_header) + 12ull * h.n + 16ull * h.n_syn + 12ull * h.n_events)
+    return fail(NC_ERR_ARG, "checkpoint size does not match its header");
+  if (h.steps_done >= (uint32_t)NC_MAX_STEPS) return fail(NC_ERR_RANGE, "checkpoint step count out of range");
+  const unsigned char* p = (const unsigned char*)buf + sizeof h;
+  NC_HIP(hipSetDevice(sim->device));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  const int32_t s = (int32_t)h.steps_done;
+  const int32_t R = sim->T.r_rel;
+  const uint32_t npos = r->nq * 64, pos0 = r->q0 * 64;
+  // neurons
+  std::vector<float> V((size_t)npos, NC_BASELEVEL);
+  std::vector<int32_t> lf((size_t)npos, NC_NEVER);
+  std::vector<uint32_t> fi((size_t)npos, 0u), asum((size_t)npos, 0u);
+  NC_HIP(hipMemcpy(V.data(), sim->V + pos0, npos * 4, hipMemcpyDeviceToHost));
+  for (uint32_t i = 0; i < h.n; ++i) {
+    const uint32_t pl = r->pos_of_id[i] - pos0;
+    memcpy(&V[pl], p, 4), memcpy(&lf[pl], p + 4, 4), memcpy(&fi[pl], p + 8, 4);
+    p += 12;
+  }
+  // synapses, and from their arrivals the active masks, summaries, pending
+  // lists and fan-out latches
+  std::vector<nc_slot> hs(r->nslots);
+  NC_HIP(hipMemcpy(hs.data(), sim->slots + r->slot0, r->nslots * sizeof(nc_slot), hipMemcpyDeviceToHost));
+  std::vector<uint32_t> masks(r->nmwords, 0u);
+  std::vector<std::vector<uint2>> pend(r->nq);
+  std::vector<int32_t> edge_arr(r->S);
+  for (uint32_t t = 0; t < h.n; ++t) {
+    const uint32_t pl = r->pos_of_id[t] - pos0, q = pl / 64, lane = pl % 64;
+    const uint32_t deg = (uint32_t)(r->in_ptr[t + 1] - r->in_ptr[t]);
+    const uint32_t nmw = (deg + 31) / 32, G = nmw <= 32 ? 1u : (nmw + 31) / 32;
+    for (uint32_t k = 0; k < deg; ++k) {
+      const uint64_t syn = (uint64_t)r->in_ptr[t] + k;
+      const unsigned char* e = p + 16ull * syn;
+      nc_slot& x = hs[r->slot_of_syn[syn] - r->slot0];
+      int32_t arr;
+      memcpy(&x.w, e, 4), memcpy(&x.depol, e + 4, 4), memcpy(&x.last_arr, e + 8, 4), memcpy(&arr, e + 12, 4);
+      x.arr_inh = arr * 2 + (x.arr_inh & 1);
+      edge_arr[syn] = arr;
+      if (arr == NC_NEVER) continue;
+      if (arr <= s && s < arr + R) {
+        const uint64_t word = sim->slice_moff_host[r->q0 + q] + (k >> 5) * 64 + lane - r->mask0;
+        masks[word] |= 1u << (k & 31);
+        asum[pl] |= 1u << ((k >> 5) / G);
+      } else if (arr > s) {
+        pend[q].push_back(make_uint2(r->slot_of_syn[syn], (uint32_t)arr));
+      }
+    }
+  }
+  p += 16ull * h.n_syn;
+  std::vector<std::pair<uint32_t, std::pair<uint32_t, uint8_t>>> events;
+  for (uint32_t i = 0; i < h.n_events; ++i) {
+    uint32_t off, id, kind;
+    memcpy(&off, p, 4), memcpy(&id, p + 4, 4), memcpy(&kind, p + 8, 4);
+    p += 12;
+    if (id >= r->n || off < 1) return fail(NC_ERR_ARG, "checkpoint event out of range");
+    events.push_back({h.steps_done + off, {r->pos_of_id[id], (uint8_t)kind}});
+  }
+  // upload
+  hipStream_t sq = sim->stream;
+  NC_HIP(hipMemcpyAsync(sim->V + pos0, V.data(), npos * 4, hipMemcpyHostToDevice, sq));
+  NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf.data(), npos * 4, hipMemcpyHostToDevice, sq));
+  NC_HIP(hipMemcpyAsync(sim->firings + pos0, fi.data(), npos * 4, hipMemcpyHostToDevice, sq));
+  NC_HIP(hipMemcpyAsync(sim->asum + pos0, asum.data(), npos * 4, hipMemcpyHostToDevice, sq));
+  NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice, sq));
+  NC_HIP(hipMemcpyAsync(sim->amask + r->mask0, masks.data(), r->nmwords * 4, hipMemcpyHostToDevice, sq));
+  std::vector<uint32_t> cnt(r->nq);
+  for (uint32_t q = 0; q < r->nq; ++q) {
+    cnt[q] = (uint32_t)pend[q].size();
+    if (cnt[q])
+      NC_HIP(hipMemcpyAsync(sim->pend + sim->slice_off_host[r->q0 + q], pend[q].data(), cnt[q] * sizeof(uint2),
+                            hipMemcpyHostToDevice, sq));
+  }
+  NC_HIP(hipMemcpyAsync(sim->pend_cnt + r->q0, cnt.data(), r->nq * 4, hipMemcpyHostToDevice, sq));
+  std::vector<nc_edge> ed(r->S);
+  if (r->S) NC_HIP(hipMemcpy(ed.data(), sim->edges + r->out0, r->S * sizeof(nc_edge), hipMemcpyDeviceToHost));
+  for (uint64_t k = 0; k < r->S; ++k) ed[r->edge_of_syn[k] - r->out0].arr = edge_arr[k];
+  if (r->S) NC_HIP(hipMemcpyAsync(sim->edges + r->out0, ed.data(), r->S * sizeof(nc_edge), hipMemcpyHostToDevice, sq));
+  NC_HIP(hipStreamSynchronize(sq));
+  r->steps_done = h.steps_done;
+  r->lr = h.lr, r->pre_f = h.pre_f, r->post_f = h.post_f;
+  r->events = events;
   return NC_OK;
 }
 

@@ -168,6 +168,12 @@ class Simulation:
         w = np.ascontiguousarray(w, np.float32)
         check(lib().nc_write_weights(self._h, net_id, ptr(w, C.c_float), len(w)))
 
+    def set_weights(self, net_id, syn, w):
+        """Synapse::setWeight on the listed synapses (export order)."""
+        syn = np.ascontiguousarray(syn, np.uint64)
+        w = np.ascontiguousarray(w, np.float32)
+        check(lib().nc_set_weights(self._h, net_id, len(syn), ptr(syn, C.c_uint64), ptr(w, C.c_float)))
+
     def synapse_state(self, net_id, S):
         arr = np.empty(S, np.int32)
         la = np.empty(S, np.int32)
@@ -212,6 +218,19 @@ class Simulation:
         check(lib().nc_sim_tune(self._h, steps_per_trial, C.byref(cfg)))
         return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
                 "graph_steps": cfg.graph_steps, "stage_tables": cfg.stage_tables}
+
+    def save(self, net_id):
+        """The network's dynamic state as bytes (a checkpoint)."""
+        size = C.c_uint64()
+        check(lib().nc_sim_save(self._h, net_id, None, 0, C.byref(size)))
+        buf = C.create_string_buffer(size.value)
+        check(lib().nc_sim_save(self._h, net_id, buf, size.value, C.byref(size)))
+        return buf.raw[:size.value]
+
+    def restore(self, net_id, data):
+        """Loads a checkpoint into a network just added from the same wiring."""
+        buf = C.create_string_buffer(bytes(data), len(data))
+        check(lib().nc_sim_restore(self._h, net_id, buf, len(data)))
 
     def set_counting(self, enable):
         check(lib().nc_set_counting(self._h, 1 if enable else 0))

@@ -85,6 +85,59 @@ float VoltageDetector::getVoltage() {
 }
 
 // ---------------------------------------------------------------------------
+// The renderer's view: Neuron and Synapse mirrors (src/NeuCor.cpp:454-579)
+
+Synapse::Synapse(NeuCor* p, std::size_t parent, std::size_t target, float w, float len)
+    : pN(parent),
+      tN(target),
+      lastSpikeStart(0.0f),
+      lastSpikeArrival(-INFINITY),
+      AP_fireTime(0.0f),
+      inhibitory(w < 0.0f),
+      parentNet(p),
+      length(len),
+      weight(w) {}
+
+void Synapse::setWeight(float w) {
+  weight = w;
+  inhibitory = weight < 0.0f;
+  written = true;
+  parentNet->weightsDirty = true;
+}
+
+float Synapse::getPrePot() const {
+  if (AP_fireTime != 0.0f) {
+    return renderPulse(float(parentNet->getTime() - lastSpikeStart) / (length * AP_speed)) * weight;
+  }
+  return 0.0f;
+}
+
+float Synapse::getPostPot() const {
+  if (AP_fireTime != 0.0f && parentNet->getTime() < AP_fireTime) {
+    return renderPulse(float(AP_fireTime - parentNet->getTime()) / (length * AP_speed)) * weight;
+  }
+  return 0.0f;
+}
+
+Neuron::Neuron(NeuCor* p, std::size_t id) : lastFire(NAN), pos(id), PA(2 * id), parentNet(p), ownID(id) {}
+
+coord3 Neuron::position() const { return parentNet->positions.at(pos); }
+float Neuron::potential() const { return parentNet->potAct.at(PA); }
+float Neuron::activity() const { return parentNet->potAct.at(PA + 1); }
+
+Synapse* NeuCor::getSynapse(std::size_t fromID, std::size_t toID) {
+  for (Synapse& s : getNeuron(fromID)->outSynapses)
+    if (s.tN == toID) return &s;
+  return nullptr;
+}
+
+const Synapse* NeuCor::getSynapse(std::size_t fromID, std::size_t toID) const {
+  for (const Synapse& s : getNeuron(fromID)->outSynapses)
+    if (s.tN == toID) return &s;
+  return nullptr;
+}
+
+// ---------------------------------------------------------------------------
 // Construction
 
 NeuCor::NeuCor(int n_neurons)
@@ -128,7 +181,7 @@ void NeuCor::createNeuron(coord3 position) {
   potAct.push_back(-70.0f);  // Neuron's baselevel (src/NeuCor.cpp:379, :394)
   potAct.push_back(0.0f);
   activityStart.push_back(currentTime);
-  out.emplace_back();
+  neurons.emplace_back(this, positions.size() - 1);
   dirty = true;
 }
 
@@ -139,7 +192,8 @@ void NeuCor::addSynapse(std::size_t from, std::size_t to, bool givenWeight, floa
   if (randomUnit() < 0.2f) weight = -weight;
   if (givenWeight) weight = w;
   const float length = positions.at(from).getDist(positions.at(to));
-  out.at(from).push_back(OutSyn{(uint32_t)to, weight, length});
+  neurons.at(from).outSynapses.emplace_back(this, from, to, weight, length);
+  neurons.at(to).inSynapses.emplace(from, to);
   dirty = true;
 }
 
@@ -149,8 +203,8 @@ void NeuCor::connectNeuron(std::size_t id) {
     const float distance = nPos.getDist(positions[i]);
     if (distance < 1.0 && i != id) {
       bool allowed = true;
-      for (const OutSyn& s : out[id])
-        if (s.to == i) {
+      for (const Synapse& s : neurons[id].outSynapses)
+        if (s.tN == i) {
           allowed = false;
           break;
         }
@@ -160,8 +214,8 @@ void NeuCor::connectNeuron(std::size_t id) {
 }
 
 void NeuCor::createSynapse(std::size_t toID, std::size_t fromID, float weight) {
-  for (const OutSyn& s : out.at(fromID))
-    if (s.to == toID) return;
+  for (const Synapse& s : neurons.at(fromID).outSynapses)
+    if (s.tN == toID) return;
   (void)positions.at(toID);
   addSynapse(fromID, toID, true, weight);
 }
@@ -213,7 +267,7 @@ void NeuCor::pullWeights() {
   // synapses created since the engine network was built keep their weights
   std::size_t f = 0;
   for (std::size_t i = 0; i < uploadedRow.size(); ++i)
-    for (std::size_t k = 0; k < uploadedRow[i]; ++k) out[i][k].weight = w[exportIndex[f++]];
+    for (std::size_t k = 0; k < uploadedRow[i]; ++k) neurons[i].outSynapses[k].weight = w[exportIndex[f++]];
 }
 
 void NeuCor::upload() {
@@ -232,9 +286,9 @@ void NeuCor::upload() {
   if (n == 0) return;
   float maxLen = 0.0f;
   std::size_t S = 0;
-  for (auto& row : out) {
-    S += row.size();
-    for (const OutSyn& s : row) maxLen = std::max(maxLen, s.length);
+  for (auto& nn : neurons) {
+    S += nn.outSynapses.size();
+    for (const Synapse& s : nn.outSynapses) maxLen = std::max(maxLen, s.length);
   }
   if (stepMs <= 0.0f) {
     // the largest power of two not above runSpeed, at most 1 ms, and coarse
@@ -249,8 +303,8 @@ void NeuCor::upload() {
   std::vector<uint8_t> d, inh;
   to.reserve(S), from.reserve(S), w.reserve(S), d.reserve(S), inh.reserve(S);
   for (uint32_t i = 0; i < n; ++i)
-    for (const OutSyn& s : out[i]) {
-      to.push_back(s.to);
+    for (const Synapse& s : neurons[i].outSynapses) {
+      to.push_back((uint32_t)s.tN);
       from.push_back(i);
       w.push_back(s.weight);
       // delay = 2 * length ms (AP_speed, src/NeuCor.cpp:485, :733), in whole steps
@@ -289,9 +343,9 @@ void NeuCor::upload() {
   delaySteps.assign(S, 0);
   std::size_t f = 0;
   for (uint32_t i = 0; i < n; ++i)
-    for (const OutSyn& s : out[i]) {
-      const uint32_t* b = in_src.data() + in_ptr[s.to];
-      const uint32_t* e = in_src.data() + in_ptr[s.to + 1];
+    for (const Synapse& s : neurons[i].outSynapses) {
+      const uint32_t* b = in_src.data() + in_ptr[s.tN];
+      const uint32_t* e = in_src.data() + in_ptr[s.tN + 1];
       const uint64_t k = (uint64_t)(std::lower_bound(b, e, i) - in_src.data());
       exportIndex[f] = k;
       delaySteps[k] = d[f];
@@ -300,7 +354,37 @@ void NeuCor::upload() {
   // a rebuilt engine starts at step 0 of the current time
   stepsDone = (uint64_t)std::floor((double)currentTime / stepMs);
   uploadedRow.resize(n);
-  for (uint32_t i = 0; i < n; ++i) uploadedRow[i] = out[i].size();
+  for (uint32_t i = 0; i < n; ++i) uploadedRow[i] = neurons[i].outSynapses.size();
+}
+
+void NeuCor::pullMirror() {
+  const std::size_t n = positions.size(), S = delaySteps.size();
+  if (!sim || n == 0) return;
+  CallerRandStream keep;
+  std::vector<int32_t> lf(n), arrival(S), lastArr(S);
+  std::vector<float> w(S), depol(S);
+  check(nc_read_last_fire(sim, netId, lf.data(), (uint32_t)n), "nc_read_last_fire");
+  if (S) {
+    check(nc_read_weights(sim, netId, w.data(), S), "nc_read_weights");
+    check(nc_read_synapse_state(sim, netId, arrival.data(), lastArr.data(), depol.data(), S),
+          "nc_read_synapse_state");
+  }
+  for (std::size_t i = 0; i < n; ++i)
+    neurons[i].lastFire = lf[i] == NC_NEVER_STEP ? NAN : (float)((double)lf[i] * stepMs);
+  std::size_t f = 0;
+  for (std::size_t i = 0; i < n; ++i)
+    for (Synapse& s : neurons[i].outSynapses) {
+      const uint64_t k = exportIndex[f++];
+      s.weight = w[k];
+      s.inhibitory = w[k] < 0.0f;
+      s.lastSpikeArrival = lastArr[k] == NC_NEVER_STEP ? -INFINITY : (float)((double)lastArr[k] * stepMs);
+      if (arrival[k] != 0) {
+        s.AP_fireTime = (float)((double)arrival[k] * stepMs);
+        s.lastSpikeStart = (float)((double)(arrival[k] - (int32_t)delaySteps[k]) * stepMs);
+      } else {
+        s.AP_fireTime = 0.0f;
+      }
+    }
 }
 
 void NeuCor::pullState() {
@@ -385,6 +469,23 @@ void NeuCor::run() {
   const uint64_t steps = target > stepsDone ? target - stepsDone : 0;
   if (sim) {
     CallerRandStream keep;
+    if (weightsDirty) {
+      // Synapse::setWeight on the mirror since the last run(), for exactly
+      // the synapses written (the others keep their sign class)
+      std::vector<uint64_t> idx;
+      std::vector<float> wv;
+      std::size_t f = 0;
+      for (std::size_t i = 0; i < uploadedRow.size(); ++i)
+        for (std::size_t k = 0; k < uploadedRow[i]; ++k, ++f) {
+          Synapse& sy = neurons[i].outSynapses[k];
+          if (!sy.written) continue;
+          idx.push_back(exportIndex[f]);
+          wv.push_back(sy.weight);
+          sy.written = false;
+        }
+      if (!idx.empty()) check(nc_set_weights(sim, netId, idx.size(), idx.data(), wv.data()), "nc_set_weights");
+      weightsDirty = false;
+    }
     if (learningRate != setLr || presynapticFactor != setPre || postsynapticFactor != setPost) {
       check(nc_set_params(sim, netId, learningRate, presynapticFactor, postsynapticFactor), "nc_set_params");
       setLr = learningRate, setPre = presynapticFactor, setPost = postsynapticFactor;
@@ -397,6 +498,7 @@ void NeuCor::run() {
   stepsDone += steps;
   currentTime = targetTime;
   pullState();
+  if (mirror) pullMirror();
 }
 
 // ---------------------------------------------------------------------------
@@ -422,8 +524,8 @@ std::vector<NeuCor::SynapseSnapshot> NeuCor::getSynapseSnapshots() const {
   }
   std::vector<SynapseSnapshot> v;
   std::size_t f = 0;
-  for (std::size_t i = 0; i < out.size(); ++i)
-    for (const OutSyn& s : out[i]) {
+  for (std::size_t i = 0; i < neurons.size(); ++i)
+    for (const Synapse& s : neurons[i].outSynapses) {
       float weight = s.weight, pre = 0.0f, post = 0.0f;
       if (live) {
         const uint64_t k = exportIndex[f];
@@ -436,7 +538,7 @@ std::vector<NeuCor::SynapseSnapshot> NeuCor::getSynapseSnapshots() const {
           if (currentTime < fireT) post = renderPulse(float(fireT - currentTime) / (s.length * 2.0f)) * weight;
         }
       }
-      v.push_back({i, s.to, positions[i], positions[s.to], weight, pre, post, weight < 0.0f});
+      v.push_back({i, s.tN, positions[i], positions[s.tN], weight, pre, post, weight < 0.0f});
       ++f;
     }
   return v;

@@ -23,6 +23,9 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <deque>
+#include <map>
+#include <utility>
 #include <vector>
 
 struct nc_net;
@@ -56,6 +59,53 @@ struct VoltageDetector {
   float radius;
   std::vector<std::size_t> near;
   float getVoltage();
+};
+
+// A synapse as the reference's renderer sees it (src/NeuCor.h:255-294): a
+// host mirror of the engine's state, refreshed after every run().
+// setWeight writes through to the engine at the next run().
+class Synapse {
+ public:
+  Synapse(NeuCor* p, std::size_t parent, std::size_t target, float weight, float length);
+  float getWeight() const { return weight; }
+  void setWeight(float w);
+
+ protected:
+  friend class NeuCor;
+  friend class NeuCor_Renderer;
+  float getPrePot() const;   // the renderer's pulse at the source end
+  float getPostPot() const;  // and at the target end
+  std::size_t pN;            // parent (source) neuron id
+  std::size_t tN;            // target neuron id
+  float lastSpikeStart;      // when the latched spike was fired (ms)
+  float lastSpikeArrival;    // when a spike last arrived (ms; -inf: never)
+  float AP_fireTime;         // arrival time of the latched spike (ms; 0: none)
+  float AP_speed = 2.0f;     // ms per unit of length
+  bool inhibitory;
+
+ private:
+  NeuCor* parentNet;
+  float length;
+  float weight;
+  bool written = false;  // setWeight since the last run()
+};
+
+// A neuron as the reference's renderer sees it (src/NeuCor.h:194-250).
+class Neuron {
+ public:
+  Neuron(NeuCor* p, std::size_t id);
+  std::vector<Synapse> outSynapses;               // owns its outgoing synapses
+  std::map<std::size_t, std::size_t> inSynapses;  // {from, to} of its incoming ones
+  coord3 position() const;
+  float potential() const;
+  float activity() const;
+  std::size_t getID() const { return ownID; }
+  float lastFire;  // simulation time of the last fire (ms; NaN: never)
+  std::size_t pos, PA;
+
+ private:
+  NeuCor* parentNet;
+  std::size_t ownID;
 };
 
 class NeuCor {
@@ -114,14 +164,29 @@ class NeuCor {
   std::vector<InputSnapshot> getInputSnapshots() const;
 
   // No counterpart in the reference: the engine's step (ms, a power of two;
-  // 0 = chosen at the first run() from runSpeed) and HIP device.
+  // 0 = chosen at the first run() from runSpeed), HIP device, and whether
+  // run() refreshes the Neuron/Synapse mirror (off: snapshots and potAct
+  // stay current, the per-synapse mirror does not).
   float stepMs = 0.0f;
   int device = 0;
+  bool mirror = true;
 
  protected:
   friend class NeuCor_Renderer;
   friend struct InputFirer;
   friend struct VoltageDetector;
+  friend class Neuron;
+  friend class Synapse;
+
+  std::deque<Neuron> neurons;  // by id
+  Neuron* getNeuron(std::size_t ID) { return &neurons.at(ID); }
+  const Neuron* getNeuron(std::size_t ID) const { return &neurons.at(ID); }
+  Synapse* getSynapse(std::size_t fromID, std::size_t toID);
+  const Synapse* getSynapse(std::size_t fromID, std::size_t toID) const;
+  Synapse* getSynapse(std::pair<std::size_t, std::size_t> ID) { return getSynapse(ID.first, ID.second); }
+  const Synapse* getSynapse(std::pair<std::size_t, std::size_t> ID) const {
+    return getSynapse(ID.first, ID.second);
+  }
 
   std::vector<coord3> positions;  // by neuron id
   std::vector<float> potAct;      // potential, activity, potential, ... (by id)
@@ -133,21 +198,17 @@ class NeuCor {
   std::vector<VoltageDetector> voltageDetectors;
 
  private:
-  struct OutSyn {
-    uint32_t to;
-    float weight;
-    float length;
-  };
   void connectNeuron(std::size_t id);  // Neuron::makeConnections (src/NeuCor.cpp:418-444)
   void addSynapse(std::size_t from, std::size_t to, bool givenWeight, float weight);
   void upload();     // (re)builds the device network from the host objects
   void pullState();  // device potentials and firings -> potAct
   void pullWeights();
+  void pullMirror();  // last fires and the synapse mirror
 
   float currentTime = 0.0f;
   uint64_t stepsDone = 0;  // grid steps simulated (time stepsDone * stepMs)
   unsigned totalGenNeurons = 0;
-  std::vector<std::vector<OutSyn>> out;  // by source id, in creation order
+  bool weightsDirty = false;  // a mirror Synapse::setWeight since the last run()
   std::vector<float> activityStart;
   std::vector<uint64_t> exportIndex;     // out-synapse (flattened, by source) -> engine synapse order
   std::vector<uint8_t> delaySteps;       // by engine synapse order

@@ -10,6 +10,10 @@
 //       runAll, runSpeed 0.02 ms) for RUNS calls of run(); prints
 //       "t TIME", "dt STEP_MS", "w FROM TO W" per synapse, "v ID V" per
 //       neuron (hex) and "r R", the next value of the caller's rand().
+//   facade_check mirror RUNS
+//       NeuCor(120) with two inputs for RUNS runs of 1 ms; the Neuron /
+//       Synapse mirror must match the snapshots, and a mirror setWeight must
+//       reach the engine (prints a summary; exit status 1 on a mismatch).
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -52,14 +56,50 @@ static int few(long runs) {
   return 0;
 }
 
+// The renderer's view after a run: every out-synapse in the Neuron mirror
+// against the snapshots, then a setWeight that must reach the engine.
+class NeuCor_Renderer {
+ public:
+  static int check(long runs) {
+    srand(3);
+    NeuCor brain(120);
+    brain.runAll = true;
+    brain.runSpeed = 1.0f;
+    float inputs[2] = {40.0f, 25.0f};
+    brain.setInputRateArray(inputs, 2);
+    for (long k = 0; k < runs; ++k) brain.run();
+    const auto snaps = brain.getSynapseSnapshots();
+    std::size_t i = 0, bad = 0, latched = 0;
+    for (auto& neu : brain.neurons)
+      for (auto& syn : neu.outSynapses) {
+        const auto& sn = snaps[i++];
+        if (sn.fromID != syn.pN || sn.toID != syn.tN || sn.weight != syn.getWeight() ||
+            sn.prePotential != syn.getPrePot() || sn.postPotential != syn.getPostPot())
+          ++bad;
+        if (syn.AP_fireTime != 0.0f) ++latched;
+        if (brain.getSynapse(syn.pN, syn.tN) != &syn) ++bad;
+        if (brain.getNeuron(syn.tN)->inSynapses.count(syn.pN) != 1) ++bad;
+      }
+    std::size_t fired = 0;
+    for (auto& neu : brain.neurons) fired += neu.lastFire == neu.lastFire;
+    Synapse* s0 = &brain.neurons[0].outSynapses[0];
+    s0->setWeight(-0.25f);
+    brain.run();
+    const float after = brain.getSynapseSnapshots()[0].weight;
+    printf("synapses %zu mismatches %zu latched %zu fired %zu weight_after_set %a\n", i, bad, latched, fired, after);
+    return bad == 0 ? 0 : 1;
+  }
+};
+
 int main(int argc, char** argv) {
   try {
     if (argc == 4 && strcmp(argv[1], "wiring") == 0) return wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
     if (argc == 3 && strcmp(argv[1], "few") == 0) return few(atol(argv[2]));
+    if (argc == 3 && strcmp(argv[1], "mirror") == 0) return NeuCor_Renderer::check(atol(argv[2]));
   } catch (const std::exception& e) {
     fprintf(stderr, "facade_check: %s\n", e.what());
     return 1;
   }
-  fprintf(stderr, "usage: facade_check wiring N SEED | few RUNS\n");
+  fprintf(stderr, "usage: facade_check wiring N SEED | few RUNS | mirror RUNS\n");
   return 2;
 }

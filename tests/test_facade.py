@@ -142,3 +142,21 @@ def test_few_neurons_through_the_facade(oracle_mod):
     # source strengthens to 1, the one driven before weakens to about 0
     # (each new pre-before-post pairing and background fire moves it a little)
     assert w[(0, 1)] == pytest.approx(1.0) and 0.0 <= w[(0, 2)] < 0.05
+
+
+@pytest.mark.gpu
+def test_renderer_mirror_matches_snapshots():
+    """The Neuron / Synapse objects the renderer reads through its friend
+    access (src/NeuCor_Renderer.cpp:655-699, :840-866) agree with the
+    snapshots after runs, and Synapse::setWeight on the mirror reaches the
+    engine."""
+    print("facade_check mirror 300", flush=True)
+    out = subprocess.run([_check_bin(), "mirror", "300"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    f = out.stdout.split()
+    vals = dict(zip(f[0::2], f[1::2]))
+    assert int(vals["synapses"]) > 0 and int(vals["mismatches"]) == 0
+    assert int(vals["fired"]) > 0
+    # the written weight, possibly moved by plasticity in the one run after it,
+    # keeps the inhibitory sign class setWeight gave it
+    assert -1.0 <= float.fromhex(vals["weight_after_set"]) <= 0.0
