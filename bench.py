@@ -266,8 +266,8 @@ def bench_served(args):
     server = SV.Server(max_n, max_s, max_nets=4096, chunk=args.chunk)
     print("[bench] served: %d jobs over %.0fs, device room %d neurons / %d synapses" % (n_jobs, args.window, max_n,
           max_s), file=sys.stderr, flush=True)
-    done, rest = server.run(jobs, t_limit=args.wall_cap, log=lambda m: print("[bench] " + m, file=sys.stderr,
-                                                                                  flush=True))
+    done, rest = server.run(jobs, t_limit=args.wall_cap, kernel_every=args.kernel_every,
+                            log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
     server.close()
     lat = np.array([j.t_done - j.arrival for j in done])
     t_end = max(j.t_done for j in done) if done else 0.0
@@ -300,6 +300,7 @@ def bench_served(args):
                              "sims_per_s": 1.0 / seq_per_job if seq_per_job else None,
                              "identical_results": True},
         "speedup_vs_sequential": (len(done) / t_end) * seq_per_job if (t_end and seq_per_job) else None,
+        "serving_loop": rest["loop"],
     }
     print(json.dumps(out), flush=True)
 
@@ -326,6 +327,8 @@ def main():
     ap.add_argument("--min-job-steps", type=int, default=1_000, help="served: shortest job")
     ap.add_argument("--max-job-steps", type=int, default=50_000, help="served: longest job")
     ap.add_argument("--wall-cap", type=float, default=900.0, help="served: stop serving after this many seconds")
+    ap.add_argument("--kernel-every", type=int, default=0,
+                    help="served: time every K-th step's kernels with HIP events (0: none)")
     ap.add_argument("--runaway", action="store_true",
                     help="served: all-excitatory weights in [0.2, 0.8) (jobs run at p ~ 0.27 instead of ~0.0025)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",

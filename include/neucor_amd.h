@@ -157,6 +157,16 @@ nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out);
 /* Exact device layout size of a network: neuron positions (a multiple of
  * 64) and synapse slots including padding, to size an nc_sim_spec. */
 nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, uint64_t* slots);
+
+/* Builds and keeps the network's device image: its layout, slot records and
+ * out-edges in network-local indices, so that every later nc_sim_add of it
+ * only relocates and uploads them.  Optional: without it nc_sim_add builds
+ * the image on each call and does not keep it.  Host work only (no device
+ * needed); different networks may be prepared on different threads at once,
+ * one network by one thread.  No counterpart in the reference, whose
+ * neurons hold their synapses directly (src/NeuCor.h:212-213); for callers
+ * that build networks off the thread that steps the simulation. */
+nc_status nc_net_prepare(nc_net* net);
 void nc_sim_destroy(nc_sim* sim);
 
 /* Bytes a network of this size takes on the device, and the free HBM the
@@ -267,12 +277,16 @@ nc_status nc_set_counting(nc_sim* sim, int enable);
  * launches; a chunk is replayed while the set of networks is unchanged);
  * stage_tables: copy the STDP trace tables into LDS per workgroup (pays when
  * many synapses see STDP per step) instead of reading them through the
- * caches.  Results do not depend on the launch shape.  Default {4, 0, 0, 0}. */
+ * caches; fanout_waves: the fan-out kernel's waves, each scanning every
+ * fanout_waves-th slice for fired neurons (never more than there are
+ * slices).  Results do not depend on the launch shape.  Default
+ * {4, 0, 0, 0, 0}; nc_sim_tune keeps fanout_waves as it finds it. */
 typedef struct nc_launch_cfg {
   uint32_t waves_per_block;
   uint32_t xcd_swizzle;
   uint32_t graph_steps;
   uint32_t stage_tables;
+  uint32_t fanout_waves; /* waves of the fan-out kernel, 64..2^20 (0 = 16384) */
 } nc_launch_cfg;
 nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg);
 nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg);

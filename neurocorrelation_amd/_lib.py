@@ -50,7 +50,7 @@ class DriveSpec(C.Structure):
 
 class LaunchCfg(C.Structure):
     _fields_ = [("waves_per_block", C.c_uint32), ("xcd_swizzle", C.c_uint32), ("graph_steps", C.c_uint32),
-                ("stage_tables", C.c_uint32)]
+                ("stage_tables", C.c_uint32), ("fanout_waves", C.c_uint32)]
 
 
 class SimSpec(C.Structure):
@@ -74,6 +74,7 @@ SIGNATURES = {
     "nc_net_select_inputs": (C.c_int, [P, C.POINTER(DriveSpec), u32p, u32p]),
     "nc_sim_create": (C.c_int, [C.POINTER(SimSpec), C.POINTER(P)]),
     "nc_net_device_size": (C.c_int, [P, u64p, u64p]),
+    "nc_net_prepare": (C.c_int, [P]),
     "nc_sim_destroy": (None, [P]),
     "nc_sim_footprint": (C.c_int, [C.c_uint64, C.c_uint64, u64p]),
     "nc_sim_free_bytes": (C.c_int, [P, u64p]),

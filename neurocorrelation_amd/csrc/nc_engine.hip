@@ -135,6 +135,143 @@ void make_layout(const nc_net& net, Layout* L) {
   }
 }
 
+}  // namespace
+
+namespace nc {
+// What a simulation keeps of a network's layout: the maps between the
+// network's export order and its device positions, slots and out-edges, all
+// network-local (a simulation adds its own first position, slot, edge).
+struct NetMaps {
+  std::vector<uint32_t> pos_of_id;    // local position per neuron id
+  std::vector<uint32_t> id_of_pos;    // neuron id per local position (nq*64, ~0u: padding)
+  std::vector<uint32_t> slot_of_syn;  // local slot per synapse, export order
+  std::vector<uint32_t> edge_of_syn;  // local out-edge per synapse, export order
+  std::vector<int64_t> in_ptr;        // incoming rows (export order)
+  uint64_t digest = 0;                // identity of the wiring (checkpoints are checked against it)
+};
+
+// A network's device image in network-local coordinates: every array
+// nc_sim_add uploads, with slot, slice and edge indices relative to the
+// network's own first slot, slice and out-edge.  nc_sim_add relocates it
+// (host offsets for the per-neuron rows, k_relocate_edges for the edges).
+struct NetImage {
+  uint32_t nq = 0;
+  uint64_t S = 0, nslots = 0, nmwords = 0, mwords_step = 0;
+  std::vector<uint32_t> slice_off, slice_moff;  // per slice: first slot, first mask word
+  std::vector<nc_slot> slots;                   // nslots records
+  std::vector<nc_edge> edges;                   // S out-edges by source position, rows in ascending target id
+  std::vector<uint2> out_row;                   // per position: (first out-edge, count)
+  std::vector<uint32_t> degw;                   // per position: in-degree | NC_DEG_VALID (0: padding)
+  std::vector<uint32_t> nid;                    // per position: neuron id (~0u: padding)
+  std::shared_ptr<const NetMaps> maps;
+};
+}  // namespace nc
+
+namespace {
+using nc::NetImage;
+using nc::NetMaps;
+
+// FNV-1a over 64-bit words (then the tail bytes).
+void fnv_mix(uint64_t* h, const void* p, size_t n) {
+  const unsigned char* c = (const unsigned char*)p;
+  uint64_t x = *h;
+  size_t i = 0;
+  for (; i + 8 <= n; i += 8) {
+    uint64_t w;
+    memcpy(&w, c + i, 8);
+    x = (x ^ w) * 1099511628211ull;
+  }
+  for (; i < n; ++i) x = (x ^ c[i]) * 1099511628211ull;
+  *h = x;
+}
+
+std::shared_ptr<const NetImage> build_image(const nc_net& net) {
+  Layout L;
+  make_layout(net, &L);
+  auto maps = std::make_shared<NetMaps>();
+  auto img = std::make_shared<NetImage>();
+  NetImage& I = *img;
+  NetMaps& M = *maps;
+  const uint64_t S = net.n_syn();
+  I.nq = L.nq;
+  I.S = S;
+  I.nslots = L.nslots;
+  I.nmwords = L.nmwords;
+  const uint32_t npos = L.nq * 64;
+  I.slice_off.resize(L.nq);
+  I.slice_moff.resize(L.nq);
+  {
+    uint64_t o = 0, mo = 0;
+    for (uint32_t q = 0; q < L.nq; ++q) {
+      I.slice_off[q] = (uint32_t)o;
+      I.slice_moff[q] = (uint32_t)mo;
+      o += L.slice_slots[q];
+      mo += L.slice_mwords[q];
+    }
+  }
+  uint64_t h = 1469598103934665603ull;
+  fnv_mix(&h, net.in_ptr.data(), net.in_ptr.size() * 8);
+  fnv_mix(&h, net.in_src.data(), net.in_src.size() * 4);
+  fnv_mix(&h, net.delay.data(), net.delay.size());
+  fnv_mix(&h, net.inh.data(), net.inh.size());
+  M.digest = h;
+  M.in_ptr = net.in_ptr;
+  // Slot records, in layout order.
+  I.slots.assign(L.nslots, nc_slot{0.0f, 0.0f, NC_NEVER, 2 * NC_NEVER});
+  M.slot_of_syn.resize(S);
+  M.edge_of_syn.assign(S, 0u);
+  for (uint32_t id = 0; id < net.n; ++id) {
+    const uint32_t pl = L.pos_of_id[id];
+    const uint32_t q = pl / 64, l = pl % 64;
+    const int64_t b = net.in_ptr[id], e = net.in_ptr[id + 1];
+    for (int64_t k = 0; k < e - b; ++k) {
+      const uint32_t j = I.slice_off[q] + (uint32_t)((k >> 2) * 256 + l * 4 + (k & 3));
+      M.slot_of_syn[b + k] = j;
+      nc_slot& x = I.slots[j];
+      x.w = net.w[b + k];
+      x.arr_inh = 2 * NC_NEVER + (net.inh[b + k] ? 1 : 0);
+    }
+  }
+  // Out-CSR by source position; rows in ascending target id.
+  std::vector<uint32_t> outcnt(npos, 0), outbeg(npos, 0);
+  for (uint64_t k = 0; k < S; ++k) outcnt[L.pos_of_id[net.in_src[k]]]++;
+  {
+    uint64_t o = 0;
+    for (uint32_t pl = 0; pl < npos; ++pl) {
+      outbeg[pl] = (uint32_t)o;
+      o += outcnt[pl];
+    }
+  }
+  I.edges.resize(S);
+  {
+    std::vector<uint32_t> fillp(outbeg);
+    for (uint32_t tg = 0; tg < net.n; ++tg) {
+      const uint32_t tq = L.pos_of_id[tg] / 64;
+      for (int64_t k = net.in_ptr[tg]; k < net.in_ptr[tg + 1]; ++k) {
+        const uint32_t e = fillp[L.pos_of_id[net.in_src[k]]]++;
+        I.edges[e] = nc_edge{M.slot_of_syn[k], (tq << 8) | net.delay[k], NC_NEVER, I.slice_off[tq]};
+        M.edge_of_syn[k] = e;
+      }
+    }
+  }
+  I.out_row.resize(npos);
+  I.degw.assign(npos, 0u);
+  I.nid.assign(npos, ~0u);
+  for (uint32_t pl = 0; pl < npos; ++pl) {
+    I.out_row[pl] = make_uint2(outbeg[pl], outcnt[pl]);
+    const uint32_t idl = L.id_of_pos[pl];
+    if (idl == ~0u) continue;
+    const uint32_t dg = (uint32_t)(net.in_ptr[idl + 1] - net.in_ptr[idl]);
+    I.degw[pl] = dg | NC_DEG_VALID;
+    I.mwords_step += (dg + 31) / 32;
+    I.nid[pl] = idl;
+  }
+  M.pos_of_id = std::move(L.pos_of_id);
+  M.id_of_pos = std::move(L.id_of_pos);
+  I.maps = maps;
+  return img;
+}
+
 struct Tables {
   float dt = 0.f, pre_decay = 0.f, post_decay = 0.f;
   std::vector<float> t_pre, t_post, w_ap;
@@ -201,13 +338,10 @@ struct NetRec {
   uint32_t q0 = 0, nq = 0;
   uint64_t slot0 = 0, nslots = 0, out0 = 0, mask0 = 0, nmwords = 0;
   uint32_t act0 = 0;  // first record of this network in the active-slice list
+  bool act_valid = false;    // the device active-slice list holds this network's records at act0
   uint64_t mwords_step = 0;  // active-mask words k_neuron reads per step
-  std::vector<uint32_t> pos_of_id;    // global positions
-  std::vector<uint32_t> slot_of_syn;  // global slot per synapse, export order
-  std::vector<uint32_t> edge_of_syn;  // global out-edge per synapse, export order
-  std::vector<int64_t> in_ptr;        // incoming rows of the network (export order)
-  uint64_t net_digest = 0;            // identity of the network's wiring
-  std::vector<uint32_t> id_of_pos;    // local, nq*64
+  std::shared_ptr<const NetMaps> maps;  // network-local layout maps
+  uint32_t pos(uint32_t id) const { return q0 * 64 + maps->pos_of_id[id]; }
   float lr = 1.f, pre_f = 0.13f, post_f = 0.30f;
   uint32_t steps_done = 0, steps_left = UINT32_MAX;
   uint32_t grp0 = 0, ngrp = 0, thr0 = 0, nthr = 0;
@@ -264,11 +398,11 @@ struct nc_sim {
   unsigned long long* ctr = nullptr;
   bool counting = false;
   uint32_t* d_chunk_off = nullptr;  // steps of the current call before the running chunk
-  nc_launch_cfg cfg{4, 0, 0, 0};
+  nc_launch_cfg cfg{4, 0, 0, 0, 0};
   // captured chunk of cfg.graph_steps steps and the launch it was captured for
   hipGraphExec_t graph = nullptr;
   nc_step_args graph_args{};
-  uint32_t graph_key[4] = {0, 0, 0, 0};
+  uint32_t graph_key[5] = {0, 0, 0, 0, 0};
   nc_netdesc* d_nets = nullptr;
   float *t_pre = nullptr, *t_post = nullptr, *w_ap = nullptr;
   DevBuf<nc_group> d_groups;
@@ -383,6 +517,7 @@ nc_status rebuild_active(nc_sim* sim) {
     NetRec& r = sim->nets[i];
     if (!r.live) continue;
     r.act0 = (uint32_t)sim->active_host.size();
+    r.act_valid = true;
     for (uint32_t q = r.q0; q < r.q0 + r.nq; ++q)
       sim->active_host.push_back(nc_slice_rec{q, sim->slice_off_host[q], (uint32_t)i, sim->slice_moff_host[q]});
   }
@@ -400,8 +535,12 @@ nc_status rebuild_active(nc_sim* sim) {
 // Pending arrivals of the last completed step get their STDP now.
 nc_status flush_net(nc_sim* sim, NetRec& r) {
   if (r.steps_done == 0 || r.nq == 0) return NC_OK;
-  nc_status st = rebuild_active(sim);
-  if (st != NC_OK) return st;
+  // The device list keeps a network's records at act0 until the next
+  // rebuild, whatever else was added or removed since.
+  if (!r.act_valid) {
+    nc_status st = rebuild_active(sim);
+    if (st != NC_OK) return st;
+  }
   nc_step_args a = base_args(sim);
   uint32_t blocks = (r.nq + 3) / 4;
   hipLaunchKernelGGL(k_flush, dim3(blocks), dim3(256), 0, sim->stream, a, r.act0, r.nq,
@@ -439,10 +578,26 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
 
 extern "C" nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, uint64_t* slots) {
   if (!net) return fail(NC_ERR_ARG, "null network");
+  if (net->image) {
+    if (positions) *positions = (uint64_t)net->image->nq * 64;
+    if (slots) *slots = net->image->nslots;
+    return NC_OK;
+  }
   Layout L;
   make_layout(*net, &L);
   if (positions) *positions = (uint64_t)L.nq * 64;
   if (slots) *slots = L.nslots;
+  return NC_OK;
+}
+
+extern "C" nc_status nc_net_prepare(nc_net* net) {
+  if (!net) return fail(NC_ERR_ARG, "null network");
+  if (net->image) return NC_OK;
+  try {
+    net->image = build_image(*net);
+  } catch (const std::bad_alloc&) {
+    return fail(NC_ERR_OOM, "host allocation failed");
+  }
   return NC_OK;
 }
 
@@ -601,24 +756,26 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     id = (int32_t)sim->nets.size() - 1;
   }
 
-  Layout L;
-  make_layout(*net, &L);
-  const uint64_t S = net->n_syn();
+  std::shared_ptr<const NetImage> img = net->image;
+  if (!img) img = build_image(*net);  // not kept: nc_net_prepare is what caches it
+  const NetImage& I = *img;
+  const NetMaps& M = *I.maps;
+  const uint64_t S = I.S;
   uint64_t q0 = 0, sl0 = 0, o0 = 0;
-  if (!sim->a_slices.alloc(L.nq, &q0)) return fail(NC_ERR_CAPACITY, "no room for the neurons");
-  if (!sim->a_slots.alloc(L.nslots / 256, &sl0)) {
-    sim->a_slices.release(q0, L.nq);
+  if (!sim->a_slices.alloc(I.nq, &q0)) return fail(NC_ERR_CAPACITY, "no room for the neurons");
+  if (!sim->a_slots.alloc(I.nslots / 256, &sl0)) {
+    sim->a_slices.release(q0, I.nq);
     return fail(NC_ERR_CAPACITY, "no room for the synapses");
   }
   if (!sim->a_out.alloc(S, &o0)) {
-    sim->a_slices.release(q0, L.nq);
-    sim->a_slots.release(sl0, L.nslots / 256);
+    sim->a_slices.release(q0, I.nq);
+    sim->a_slots.release(sl0, I.nslots / 256);
     return fail(NC_ERR_CAPACITY, "no room for the out-synapse lists");
   }
   uint64_t mk0 = 0;
-  if (!sim->a_masks.alloc(L.nmwords / 64, &mk0)) {
-    sim->a_slices.release(q0, L.nq);
-    sim->a_slots.release(sl0, L.nslots / 256);
+  if (!sim->a_masks.alloc(I.nmwords / 64, &mk0)) {
+    sim->a_slices.release(q0, I.nq);
+    sim->a_slots.release(sl0, I.nslots / 256);
     sim->a_out.release(o0, S);
     return fail(NC_ERR_CAPACITY, "no room for the active-slot masks");
   }
@@ -630,97 +787,31 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   r.n = net->n;
   r.S = S;
   r.q0 = (uint32_t)q0;
-  r.nq = L.nq;
+  r.nq = I.nq;
   r.slot0 = sl0;
-  r.nslots = L.nslots;
+  r.nslots = I.nslots;
   r.out0 = o0;
   r.mask0 = mk0;
-  r.nmwords = L.nmwords;
+  r.nmwords = I.nmwords;
   r.lr = p.learning_rate;
   r.pre_f = p.pre_factor;
   r.post_f = p.post_factor;
   r.steps_left = steps_to_run ? steps_to_run : UINT32_MAX;
-  r.id_of_pos = L.id_of_pos;
-  r.pos_of_id.resize(net->n);
-  for (uint32_t i = 0; i < net->n; ++i) r.pos_of_id[i] = (uint32_t)(q0 * 64) + L.pos_of_id[i];
+  r.maps = I.maps;
+  r.mwords_step = I.mwords_step;
 
-  const uint32_t npos = L.nq * 64;
+  const uint32_t npos = I.nq * 64;
   const uint32_t pos0 = (uint32_t)(q0 * 64);
-  std::vector<uint32_t> slice_off(L.nq), slice_moff(L.nq);
-  {
-    uint64_t o = sl0, mo = mk0;
-    for (uint32_t q = 0; q < L.nq; ++q) {
-      slice_off[q] = (uint32_t)o;
-      slice_moff[q] = (uint32_t)mo;
-      o += L.slice_slots[q];
-      mo += L.slice_mwords[q];
-    }
+  std::vector<uint32_t> slice_off(I.nq), slice_moff(I.nq);
+  for (uint32_t q = 0; q < I.nq; ++q) {
+    slice_off[q] = (uint32_t)sl0 + I.slice_off[q];
+    slice_moff[q] = (uint32_t)mk0 + I.slice_moff[q];
   }
-  // Slot records, in layout order.
-  std::vector<nc_slot> hs(L.nslots);
-  for (auto& x : hs) x = nc_slot{0.0f, 0.0f, NC_NEVER, 2 * NC_NEVER};
-  r.slot_of_syn.resize(S);
-  r.edge_of_syn.assign(S, 0u);
-  r.in_ptr = net->in_ptr;
-  {
-    // FNV-1a over the wiring: the identity a checkpoint is checked against
-    uint64_t h = 1469598103934665603ull;
-    auto mix = [&h](const void* p, size_t n) {
-      const unsigned char* c = (const unsigned char*)p;
-      for (size_t i = 0; i < n; ++i) h = (h ^ c[i]) * 1099511628211ull;
-    };
-    mix(net->in_ptr.data(), net->in_ptr.size() * 8);
-    mix(net->in_src.data(), net->in_src.size() * 4);
-    mix(net->delay.data(), net->delay.size());
-    mix(net->inh.data(), net->inh.size());
-    r.net_digest = h;
-  }
-  for (uint32_t id_ = 0; id_ < net->n; ++id_) {
-    const uint32_t pl = L.pos_of_id[id_];
-    const uint32_t q = pl / 64, l = pl % 64;
-    const int64_t b = net->in_ptr[id_], e = net->in_ptr[id_ + 1];
-    for (int64_t k = 0; k < e - b; ++k) {
-      uint32_t j = slice_off[q] + (uint32_t)((k >> 2) * 256 + l * 4 + (k & 3));
-      r.slot_of_syn[b + k] = j;
-      nc_slot& x = hs[j - sl0];
-      x.w = net->w[b + k];
-      x.arr_inh = 2 * NC_NEVER + (net->inh[b + k] ? 1 : 0);
-    }
-  }
-  // Out-CSR by source position; rows in ascending target id.
-  std::vector<uint32_t> outcnt(npos, 0), outbeg(npos, 0);
-  for (uint64_t k = 0; k < S; ++k) outcnt[L.pos_of_id[net->in_src[k]]]++;
-  {
-    uint64_t o = o0;
-    for (uint32_t pl = 0; pl < npos; ++pl) {
-      outbeg[pl] = (uint32_t)o;
-      o += outcnt[pl];
-    }
-  }
-  std::vector<nc_edge> oedge(S);
-  {
-    std::vector<uint32_t> fillp(outbeg);
-    for (uint32_t tg = 0; tg < net->n; ++tg) {
-      for (int64_t k = net->in_ptr[tg]; k < net->in_ptr[tg + 1]; ++k) {
-        uint32_t sp = L.pos_of_id[net->in_src[k]];
-        uint32_t e = fillp[sp]++ - (uint32_t)o0;
-        const uint32_t tq = (uint32_t)q0 + L.pos_of_id[tg] / 64;
-        oedge[e] = nc_edge{r.slot_of_syn[k], (tq << 8) | net->delay[k], NC_NEVER, slice_off[tq - q0]};
-        r.edge_of_syn[k] = (uint32_t)o0 + e;
-      }
-    }
-  }
+  std::vector<uint2> orow(npos);
+  for (uint32_t pl = 0; pl < npos; ++pl) orow[pl] = make_uint2((uint32_t)o0 + I.out_row[pl].x, I.out_row[pl].y);
   // Drive.
-  std::vector<uint32_t> ingrp(npos, 0), degw(npos, 0), nidv(npos, ~0u);
-  r.mwords_step = 0;
-  for (uint32_t pl = 0; pl < npos; ++pl) {
-    uint32_t idl = L.id_of_pos[pl];
-    if (idl == ~0u) continue;
-    const uint32_t dg = (uint32_t)(net->in_ptr[idl + 1] - net->in_ptr[idl]);
-    degw[pl] = dg | NC_DEG_VALID;
-    r.mwords_step += (dg + 31) / 32;
-    nidv[pl] = idl;
-  }
+  std::vector<uint2> ndwv(npos);
+  for (uint32_t pl = 0; pl < npos; ++pl) ndwv[pl] = make_uint2(I.degw[pl], 0u);
   nc_netdesc nd{};
   nd.lr = r.lr;
   nd.pre_f = r.pre_f;
@@ -729,10 +820,10 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     nc::InputSel sel;
     nc_status stt = nc::select_inputs(*net, *drive, &sel);
     if (stt != NC_OK) {
-      sim->a_slices.release(q0, L.nq);
-      sim->a_slots.release(sl0 / 256, L.nslots / 256);
+      sim->a_slices.release(q0, I.nq);
+      sim->a_slots.release(sl0 / 256, I.nslots / 256);
       sim->a_out.release(o0, S);
-      sim->a_masks.release(mk0 / 64, L.nmwords / 64);
+      sim->a_masks.release(mk0 / 64, I.nmwords / 64);
       r.live = false;
       return stt;
     }
@@ -762,9 +853,9 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     }
     r.nthr = (uint32_t)sim->thr.size() - r.thr0;
     for (size_t k = 0; k < sel.ids.size(); ++k) {
-      uint32_t pl = L.pos_of_id[sel.ids[k]];
-      degw[pl] |= NC_DEG_INPUT;
-      ingrp[pl] = r.grp0 + sel.group[k];
+      uint32_t pl = M.pos_of_id[sel.ids[k]];
+      ndwv[pl].x |= NC_DEG_INPUT;
+      ndwv[pl].y = r.grp0 + sel.group[k];
     }
     nd.drive_lo = (uint32_t)drive->seed;
     nd.drive_hi = (uint32_t)(drive->seed >> 32);
@@ -780,28 +871,30 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   }
   std::vector<float> V0(npos, NC_BASELEVEL);
   std::vector<int32_t> lf0(npos, NC_NEVER);
-  std::vector<uint32_t> zero(npos, 0), snet(L.nq, (uint32_t)id);
+  std::vector<uint32_t> zero(npos, 0), snet(I.nq, (uint32_t)id);
   hipStream_t st = sim->stream;
-  NC_HIP(hipMemcpyAsync(sim->slots + sl0, hs.data(), L.nslots * sizeof(nc_slot), hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemsetAsync(sim->amask + mk0, 0, L.nmwords * 4, st));
+  NC_HIP(hipMemcpyAsync(sim->slots + sl0, I.slots.data(), I.nslots * sizeof(nc_slot), hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemsetAsync(sim->amask + mk0, 0, I.nmwords * 4, st));
   NC_HIP(hipMemsetAsync(sim->asum + pos0, 0, npos * 4, st));
   NC_HIP(hipMemcpyAsync(sim->V + pos0, V0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf0.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->firings + pos0, zero.data(), npos * 4, hipMemcpyHostToDevice, st));
-  std::vector<uint2> ndwv(npos);
-  for (uint32_t pl = 0; pl < npos; ++pl) ndwv[pl] = make_uint2(degw[pl], ingrp[pl]);
   NC_HIP(hipMemcpyAsync(sim->ndw + pos0, ndwv.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->nid + pos0, nidv.data(), npos * 4, hipMemcpyHostToDevice, st));
-  std::vector<uint2> orow(npos);
-  for (uint32_t pl = 0; pl < npos; ++pl) orow[pl] = make_uint2(outbeg[pl], outcnt[pl]);
+  NC_HIP(hipMemcpyAsync(sim->nid + pos0, I.nid.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->out_row + pos0, orow.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
-  if (S) NC_HIP(hipMemcpyAsync(sim->edges + o0, oedge.data(), S * sizeof(nc_edge), hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, L.nq * 4, st));
-  NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), L.nq * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), L.nq * 4, hipMemcpyHostToDevice, st));
+  if (S) {
+    NC_HIP(hipMemcpyAsync(sim->edges + o0, I.edges.data(), S * sizeof(nc_edge), hipMemcpyHostToDevice, st));
+    const uint32_t rb = (uint32_t)std::min<uint64_t>((S + 255) / 256, 4096);
+    hipLaunchKernelGGL(k_relocate_edges, dim3(rb), dim3(256), 0, st, sim->edges + o0, S, (uint32_t)sl0,
+                       (uint32_t)q0);
+    NC_HIP(hipGetLastError());
+  }
+  NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * 4, st));
+  NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), I.nq * 4, hipMemcpyHostToDevice, st));
+  NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
   NC_HIP(hipStreamSynchronize(st));
-  for (uint32_t q = 0; q < L.nq; ++q) {
+  for (uint32_t q = 0; q < I.nq; ++q) {
     sim->slice_net_host[q0 + q] = (uint32_t)id;
     sim->slice_off_host[q0 + q] = slice_off[q];
     sim->slice_moff_host[q0 + q] = slice_moff[q];
@@ -828,8 +921,7 @@ extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) {
   // Drive groups stay allocated in the host vectors (append-only); they are
   // compacted when no network uses them.
   r->live = false;
-  r->pos_of_id.clear();
-  r->slot_of_syn.clear();
+  r->maps.reset();
   sim->active_dirty = true;
   bool any = false;
   for (auto& x : sim->nets) any |= x.live;
@@ -956,6 +1048,10 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
   }
   const uint32_t wpb = sim->cfg.waves_per_block;
   const uint32_t grid = (a.n_active_slices + wpb - 1) / wpb;
+  // k_fanout scans the fired records with at most fanout_waves waves (default
+  // 16384, 64 per CU): a step's fired slices are few.
+  const uint32_t fan_waves = sim->cfg.fanout_waves ? sim->cfg.fanout_waves : 16384u;
+  const uint32_t fan_grid = std::min<uint32_t>(grid, (fan_waves + wpb - 1) / wpb);
   const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
   const void* fn_neuron = nullptr;
   const void* fn_fanout = nullptr;
@@ -965,7 +1061,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     case 8: fn_neuron = (const void*)k_neuron<8>, fn_fanout = (const void*)k_fanout<8>; break;
     default: fn_neuron = (const void*)k_neuron<4>, fn_fanout = (const void*)k_fanout<4>; break;
   }
-  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, sim->d_chunk_off, 0u);
+  hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, sim->d_chunk_off, 1u, 0u);
   NC_HIP(hipGetLastError());
   size_t nev_used = 0;
   std::vector<int> timed_kinds;
@@ -992,7 +1088,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
   const uint32_t G = sim->cfg.graph_steps;
   if (G && ev_pos.empty() && sim->kernel_timing == 0 && max_run >= G) {
     a.chunk_i = 0;
-    const uint32_t key[4] = {G, wpb, grid, lds};
+    const uint32_t key[5] = {G, wpb, grid, lds, fan_grid};
     if (!sim->graph || memcmp(&sim->graph_args, &a, sizeof a) != 0 || memcmp(sim->graph_key, key, sizeof key) != 0) {
       if (sim->graph) {
         (void)hipGraphExecDestroy(sim->graph);
@@ -1004,7 +1100,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
         a.chunk_i = (int32_t)c;
         void* args[] = {&a};
         if (hipLaunchKernel(fn_neuron, dim3(grid), dim3(64 * wpb), args, lds, st) != hipSuccess ||
-            hipLaunchKernel(fn_fanout, dim3(grid), dim3(64 * wpb), args, 0, st) != hipSuccess)
+            hipLaunchKernel(fn_fanout, dim3(fan_grid), dim3(64 * wpb), args, 0, st) != hipSuccess)
           cs = fail(NC_ERR_HIP, "kernel launch during capture failed");
       }
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, sim->d_chunk_off, G);
@@ -1037,7 +1133,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
       if (e != hipSuccess) s1 = fail(NC_ERR_HIP, hipGetErrorString(e));
     }
     if (s1 == NC_OK) s1 = launch(1, fn_neuron, grid, 64 * wpb, lds);
-    if (s1 == NC_OK) s1 = launch(3, fn_fanout, grid, 64 * wpb, 0);
+    if (s1 == NC_OK) s1 = launch(3, fn_fanout, fan_grid, 64 * wpb, 0);
     if (s1 != NC_OK) return s1;
     sim->timing_step++;
   }
@@ -1109,7 +1205,7 @@ extern "C" nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_even
     if (kind[k] > 1) return fail(NC_ERR_ARG, "kind must be 0 (input) or 1 (background)");
   }
   for (uint32_t k = 0; k < n_events; ++k)
-    r->events.push_back({r->steps_done + step_off[k], {r->pos_of_id[ids[k]], kind[k]}});
+    r->events.push_back({r->steps_done + step_off[k], {r->pos(ids[k]), kind[k]}});
   return NC_OK;
 }
 
@@ -1123,7 +1219,7 @@ nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t 
                         sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
   const uint32_t p0 = r.q0 * 64;
-  for (uint32_t i = 0; i < n; ++i) out[i] = tmp[r.pos_of_id[i] - p0];
+  for (uint32_t i = 0; i < n; ++i) out[i] = tmp[r.maps->pos_of_id[i]];
   return NC_OK;
 }
 
@@ -1172,7 +1268,7 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   std::vector<nc_slot> hs;
   st = read_slots(sim, *r, &hs);
   if (st != NC_OK) return st;
-  for (uint64_t k = 0; k < n; ++k) out[k] = hs[r->slot_of_syn[k] - r->slot0].w;
+  for (uint64_t k = 0; k < n; ++k) out[k] = hs[r->maps->slot_of_syn[k]].w;
   return NC_OK;
 }
 
@@ -1187,7 +1283,7 @@ extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* 
   if (st != NC_OK) return st;
   // Synapse::setWeight also re-derives the sign class (src/NeuCor.cpp:576-579).
   for (uint64_t k = 0; k < n; ++k) {
-    nc_slot& x = hs[r->slot_of_syn[k] - r->slot0];
+    nc_slot& x = hs[r->maps->slot_of_syn[k]];
     x.w = w[k];
     x.arr_inh = (x.arr_inh & ~1) | ((w[k] < 0.0f) ? 1 : 0);
   }
@@ -1208,7 +1304,7 @@ extern "C" nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, con
   st = read_slots(sim, *r, &hs);
   if (st != NC_OK) return st;
   for (uint64_t i = 0; i < n; ++i) {
-    nc_slot& x = hs[r->slot_of_syn[syn[i]] - r->slot0];
+    nc_slot& x = hs[r->maps->slot_of_syn[syn[i]]];
     x.w = w[i];
     x.arr_inh = (x.arr_inh & ~1) | ((w[i] < 0.0f) ? 1 : 0);
   }
@@ -1231,7 +1327,7 @@ extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t*
   // releases (R steps after arrival), then 0.
   const int32_t s = (int32_t)r->steps_done;
   for (uint64_t k = 0; k < n; ++k) {
-    const uint64_t j = r->slot_of_syn[k] - r->slot0;
+    const uint64_t j = r->maps->slot_of_syn[k];
     const int32_t a = hs[j].arr_inh >> 1;
     if (arrival) arrival[k] = (a != NC_NEVER && s < a + sim->T.r_rel) ? a : 0;
     if (last_arrival) last_arrival[k] = hs[j].last_arr;
@@ -1259,7 +1355,7 @@ extern "C" nc_status nc_read_raster(nc_sim* sim, int32_t net_id, uint32_t* steps
   const uint32_t p0 = r->q0 * 64, p1 = p0 + r->nq * 64;
   for (uint32_t k = 0; k < cnt; ++k) {
     uint32_t pos = buf[2 * k], s = buf[2 * k + 1];
-    if (pos >= p0 && pos < p1) mine.push_back({s, r->id_of_pos[pos - p0]});
+    if (pos >= p0 && pos < p1) mine.push_back({s, r->maps->id_of_pos[pos - p0]});
     else rest.push_back({pos, s});
   }
   std::sort(mine.begin(), mine.end());
@@ -1321,7 +1417,7 @@ extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_
   h.steps_done = r->steps_done;
   h.lr = r->lr, h.pre_f = r->pre_f, h.post_f = r->post_f, h.dt = sim->T.dt;
   h.n_events = (uint32_t)r->events.size();
-  h.digest = r->net_digest;
+  h.digest = r->maps->digest;
   memcpy(p, &h, sizeof h);
   p += sizeof h;
   std::vector<float> V(r->n);
@@ -1336,14 +1432,14 @@ extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_
     p += 12;
   }
   for (uint64_t k = 0; k < r->S; ++k) {
-    const nc_slot& x = hs[r->slot_of_syn[k] - r->slot0];
+    const nc_slot& x = hs[r->maps->slot_of_syn[k]];
     const int32_t arr = x.arr_inh >> 1;
     memcpy(p, &x.w, 4), memcpy(p + 4, &x.depol, 4), memcpy(p + 8, &x.last_arr, 4), memcpy(p + 12, &arr, 4);
     p += 16;
   }
   for (auto& ev : r->events) {
     const uint32_t off = ev.first - r->steps_done, pos = ev.second.first - r->q0 * 64;
-    const uint32_t id = r->id_of_pos[pos], kind = ev.second.second;
+    const uint32_t id = r->maps->id_of_pos[pos], kind = ev.second.second;
     memcpy(p, &off, 4), memcpy(p + 4, &id, 4), memcpy(p + 8, &kind, 4);
     p += 12;
   }
@@ -1358,7 +1454,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   nc_ckpt_header h;
   memcpy(&h, buf, sizeof h);
   if (h.magic != 0x314b434eu || h.version != 1) return fail(NC_ERR_ARG, "not a checkpoint (magic/version)");
-  if (h.n != r->n || h.n_syn != r->S || h.digest != r->net_digest)
+  if (h.n != r->n || h.n_syn != r->S || h.digest != r->maps->digest)
     return fail(NC_ERR_ARG, "checkpoint of a different network");
   if (h.dt != sim->T.dt) return fail(NC_ERR_ARG, "checkpoint taken at another dt");
   if (r->steps_done != 0) return fail(NC_ERR_STATE, "restore into a network that has not stepped yet");
@@ -1377,7 +1473,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   std::vector<uint32_t> fi((size_t)npos, 0u), asum((size_t)npos, 0u);
   NC_HIP(hipMemcpy(V.data(), sim->V + pos0, npos * 4, hipMemcpyDeviceToHost));
   for (uint32_t i = 0; i < h.n; ++i) {
-    const uint32_t pl = r->pos_of_id[i] - pos0;
+    const uint32_t pl = r->maps->pos_of_id[i];
     memcpy(&V[pl], p, 4), memcpy(&lf[pl], p + 4, 4), memcpy(&fi[pl], p + 8, 4);
     p += 12;
   }
@@ -1389,13 +1485,13 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   std::vector<std::vector<uint2>> pend(r->nq);
   std::vector<int32_t> edge_arr(r->S);
   for (uint32_t t = 0; t < h.n; ++t) {
-    const uint32_t pl = r->pos_of_id[t] - pos0, q = pl / 64, lane = pl % 64;
-    const uint32_t deg = (uint32_t)(r->in_ptr[t + 1] - r->in_ptr[t]);
+    const uint32_t pl = r->maps->pos_of_id[t], q = pl / 64, lane = pl % 64;
+    const uint32_t deg = (uint32_t)(r->maps->in_ptr[t + 1] - r->maps->in_ptr[t]);
     const uint32_t nmw = (deg + 31) / 32, G = nmw <= 32 ? 1u : (nmw + 31) / 32;
     for (uint32_t k = 0; k < deg; ++k) {
-      const uint64_t syn = (uint64_t)r->in_ptr[t] + k;
+      const uint64_t syn = (uint64_t)r->maps->in_ptr[t] + k;
       const unsigned char* e = p + 16ull * syn;
-      nc_slot& x = hs[r->slot_of_syn[syn] - r->slot0];
+      nc_slot& x = hs[r->maps->slot_of_syn[syn]];
       int32_t arr;
       memcpy(&x.w, e, 4), memcpy(&x.depol, e + 4, 4), memcpy(&x.last_arr, e + 8, 4), memcpy(&arr, e + 12, 4);
       x.arr_inh = arr * 2 + (x.arr_inh & 1);
@@ -1406,7 +1502,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
         masks[word] |= 1u << (k & 31);
         asum[pl] |= 1u << ((k >> 5) / G);
       } else if (arr > s) {
-        pend[q].push_back(make_uint2(r->slot_of_syn[syn], (uint32_t)arr));
+        pend[q].push_back(make_uint2((uint32_t)r->slot0 + r->maps->slot_of_syn[syn], (uint32_t)arr));
       }
     }
   }
@@ -1417,7 +1513,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
     memcpy(&off, p, 4), memcpy(&id, p + 4, 4), memcpy(&kind, p + 8, 4);
     p += 12;
     if (id >= r->n || off < 1) return fail(NC_ERR_ARG, "checkpoint event out of range");
-    events.push_back({h.steps_done + off, {r->pos_of_id[id], (uint8_t)kind}});
+    events.push_back({h.steps_done + off, {r->pos(id), (uint8_t)kind}});
   }
   // upload
   hipStream_t sq = sim->stream;
@@ -1437,7 +1533,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipMemcpyAsync(sim->pend_cnt + r->q0, cnt.data(), r->nq * 4, hipMemcpyHostToDevice, sq));
   std::vector<nc_edge> ed(r->S);
   if (r->S) NC_HIP(hipMemcpy(ed.data(), sim->edges + r->out0, r->S * sizeof(nc_edge), hipMemcpyDeviceToHost));
-  for (uint64_t k = 0; k < r->S; ++k) ed[r->edge_of_syn[k] - r->out0].arr = edge_arr[k];
+  for (uint64_t k = 0; k < r->S; ++k) ed[r->maps->edge_of_syn[k]].arr = edge_arr[k];
   if (r->S) NC_HIP(hipMemcpyAsync(sim->edges + r->out0, ed.data(), r->S * sizeof(nc_edge), hipMemcpyHostToDevice, sq));
   NC_HIP(hipStreamSynchronize(sq));
   r->steps_done = h.steps_done;
@@ -1483,6 +1579,8 @@ extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) {
   if (cfg->xcd_swizzle > 1) return fail(NC_ERR_ARG, "xcd_swizzle must be 0 or 1");
   if (cfg->graph_steps > 4096) return fail(NC_ERR_ARG, "graph_steps must be at most 4096");
   if (cfg->stage_tables > 1) return fail(NC_ERR_ARG, "stage_tables must be 0 or 1");
+  if (cfg->fanout_waves && (cfg->fanout_waves < 64 || cfg->fanout_waves > (1u << 20)))
+    return fail(NC_ERR_ARG, "fanout_waves must be 0 or in [64, 2^20]");
   sim->cfg = *cfg;
   return NC_OK;
 }
@@ -1511,7 +1609,7 @@ extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launc
   for (uint32_t g : graphs)
     for (uint32_t w : waves)
       for (uint32_t x = 0; x < 4 && st == NC_OK; ++x) {
-        sim->cfg = nc_launch_cfg{w, x & 1u, g, x >> 1};
+        sim->cfg = nc_launch_cfg{w, x & 1u, g, x >> 1, start.fanout_waves};
         st = nc_sim_step(sim, steps_per_trial);  // warm-up (and graph capture)
         if (st != NC_OK) break;
         if (hipEventRecord(e0, sim->stream) != hipSuccess) {

@@ -1,6 +1,7 @@
 // Host-side internals shared by the network builder and the engine.
 #pragma once
 #include <stdint.h>
+#include <memory>
 #include <string>
 #include <vector>
 
@@ -11,6 +12,10 @@
 // order of Neuron::inSynapses (std::map keyed by source, src/NeuCor.h:212),
 // which fixes the float summation order of charge_insynapses
 // (src/NeuCor.cpp:688-700).
+namespace nc {
+struct NetImage;  // the engine's device image of a network (nc_net_prepare)
+}
+
 struct nc_net {
   uint32_t n = 0;
   std::vector<float> xyz;          // 3 per neuron
@@ -21,6 +26,9 @@ struct nc_net {
   std::vector<uint8_t> inh;        // S, 1 = inhibitory sign class
   uint32_t max_in_degree = 0;
   uint32_t max_delay = 0;
+  // Cached device image, built by nc_net_prepare; a network never changes
+  // after construction, so the image stays valid for its lifetime.
+  std::shared_ptr<const nc::NetImage> image;
   uint64_t n_syn() const { return in_src.size(); }
 };
 

@@ -177,8 +177,22 @@ __device__ __forceinline__ uint32_t nc_block_id(uint32_t swizzle) {
 __global__ void k_advance(uint32_t* chunk_off, uint32_t steps) {
   if (threadIdx.x == 0) *chunk_off += steps;
 }
-__global__ void k_set_u32(uint32_t* p, uint32_t v) {
-  if (threadIdx.x == 0) *p = v;
+__global__ void k_set_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  if (threadIdx.x < n) p[threadIdx.x] = v;
+}
+
+// Moves a network's out-edges from network-local to simulation indices:
+// target slot and slice base by the network's first slot, target slice by
+// its first slice (tq_d = slice << 8 | delay).
+__global__ __launch_bounds__(256) void k_relocate_edges(nc_edge* e, uint64_t n, uint32_t slot0, uint32_t q0) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    nc_edge x = e[i];
+    x.j += slot0;
+    x.tq_d += q0 << 8;
+    x.tbase += slot0;
+    e[i] = x;
+  }
 }
 
 // Slot index of in-slot k of lane l in a slice whose slots start at slot_off
@@ -381,7 +395,7 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
   if (wave_in && lane == 0)
-    a.fired[wave] = live ? make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q) : make_uint4(0u, 0u, 0u, 0u);
+    a.fired[wave] = fm ? make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q) : make_uint4(0u, 0u, 0u, 0u);
   if (fm) {
     // The row passes below read the fired lanes' slot records from other
     // lanes; when a fired lane wrote some of them above (arrival STDP), make
@@ -470,8 +484,9 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
 }
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
-// fired neuron: one wave per active slice (most leave after one load), one
-// writer per synapse (its source), latch permitting.  Accepted spikes latch
+// fired neuron: each wave of the grid scans every nwaves-th fired record of
+// the active slices, 64 per load, and takes the slices with fired lanes one
+// by one; one writer per synapse (its source), latch permitting.  Accepted spikes latch
 // depolarisation and arrival step into the target slot and join the target
 // slice's pending list.
 __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, const nc_edge& ed, int32_t s,
@@ -493,52 +508,63 @@ __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
   const uint32_t blk = nc_block_id(a.xcd_swizzle);
   const uint32_t wave = blk * WPB + (threadIdx.x >> 6);
   const uint32_t lane = threadIdx.x & 63;
-  uint32_t n_acc = 0;
-  const uint4 f = wave < a.n_active_slices ? a.fired[wave] : make_uint4(0u, 0u, 0u, 0u);
-  const unsigned long long m = ((unsigned long long)f.y << 32) | f.x;
-  uint32_t total = 0;
-  if (m) {
-    // The out-lists of all fired lanes, flattened: lane-parallel row loads,
-    // a wave prefix sum of out-degrees, then 128 edges per pass.
-    const int32_t s = (int32_t)f.z;
-    uint2 row = make_uint2(0u, 0u);
-    if ((m >> lane) & 1ull) row = a.out_row[f.w * 64 + lane];
-    uint32_t incl = row.y;
+  const uint32_t nwaves = gridDim.x * WPB;
+  uint32_t n_acc = 0, n_out = 0;
+  // Interleaved: this wave's slices are wave, wave + nwaves, ... (a busy
+  // stretch of slices spreads over all waves), 64 records per load.
+  for (uint32_t fb = wave; fb < a.n_active_slices; fb += 64 * nwaves) {
+    const uint32_t fi = fb + lane * nwaves;
+    const uint4 fl = fi < a.n_active_slices ? a.fired[fi] : make_uint4(0u, 0u, 0u, 0u);
+    unsigned long long has = __ballot((fl.x | fl.y) != 0u);
+    while (has) {
+      const uint32_t fsrc = (uint32_t)__builtin_ctzll(has);
+      has &= has - 1;
+      const uint32_t fx = (uint32_t)__shfl((int)fl.x, (int)fsrc), fy = (uint32_t)__shfl((int)fl.y, (int)fsrc);
+      const int32_t s = __shfl((int)fl.z, (int)fsrc);
+      const uint32_t fq = (uint32_t)__shfl((int)fl.w, (int)fsrc);
+      const unsigned long long m = ((unsigned long long)fy << 32) | fx;
+      // The out-lists of all fired lanes, flattened: lane-parallel row loads,
+      // a wave prefix sum of out-degrees, then 128 edges per pass.
+      uint2 row = make_uint2(0u, 0u);
+      if ((m >> lane) & 1ull) row = a.out_row[fq * 64 + lane];
+      uint32_t incl = row.y;
 #pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
-      if (lane >= (uint32_t)o) incl += v;
-    }
-    total = (uint32_t)__shfl((int)incl, 63);
-    for (uint32_t base = 0; base < total; base += 128) {
-      uint32_t ie[2], src[2];
-      nc_edge ed[2];
-#pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const uint32_t i = base + 64u * h + lane;
-        // source lane: the first lane whose inclusive prefix exceeds i
-        uint32_t lo = 0;
-#pragma unroll
-        for (uint32_t b = 32; b >= 1; b >>= 1) {
-          const uint32_t v = (uint32_t)__shfl((int)incl, (int)(lo + b - 1));
-          if (v <= i) lo += b;
-        }
-        src[h] = lo > 63 ? 63 : lo;
-        const uint32_t rx = (uint32_t)__shfl((int)row.x, (int)src[h]);
-        const uint32_t ry = (uint32_t)__shfl((int)row.y, (int)src[h]);
-        const uint32_t ic = (uint32_t)__shfl((int)incl, (int)src[h]);
-        ie[h] = i < total ? rx + (i - (ic - ry)) : ~0u;
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, o);
+        if (lane >= (uint32_t)o) incl += v;
       }
+      const uint32_t total = (uint32_t)__shfl((int)incl, 63);
+      n_out += total;
+      for (uint32_t base = 0; base < total; base += 128) {
+        uint32_t ie[2], src[2];
+        nc_edge ed[2];
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        if (ie[h] != ~0u) ed[h] = a.edges[ie[h]];
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t i = base + 64u * h + lane;
+          // source lane: the first lane whose inclusive prefix exceeds i
+          uint32_t lo = 0;
 #pragma unroll
-      for (int h = 0; h < 2; ++h)
-        if (ie[h] != ~0u) nc_fire_edge(a, ie[h], ed[h], s, n_acc);
+          for (uint32_t b = 32; b >= 1; b >>= 1) {
+            const uint32_t v = (uint32_t)__shfl((int)incl, (int)(lo + b - 1));
+            if (v <= i) lo += b;
+          }
+          src[h] = lo > 63 ? 63 : lo;
+          const uint32_t rx = (uint32_t)__shfl((int)row.x, (int)src[h]);
+          const uint32_t ry = (uint32_t)__shfl((int)row.y, (int)src[h]);
+          const uint32_t ic = (uint32_t)__shfl((int)incl, (int)src[h]);
+          ie[h] = i < total ? rx + (i - (ic - ry)) : ~0u;
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (ie[h] != ~0u) ed[h] = a.edges[ie[h]];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          if (ie[h] != ~0u) nc_fire_edge(a, ie[h], ed[h], s, n_acc);
+      }
     }
   }
   if (a.ctr) {
-    if (lane == 0 && total) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 6], (unsigned long long)total);
+    if (lane == 0 && n_out) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 6], (unsigned long long)n_out);
     nc_count32(a.ctr, blk, 7, n_acc);
   }
 }

@@ -13,6 +13,10 @@ from ._lib import DriveSpec, LaunchCfg, Params, SimSpec, WiringSpec, check, lib,
 NEVER = _lib.NC_NEVER_STEP
 
 
+def _cfg_dict(cfg):
+    return {f: getattr(cfg, f) for f, _ in LaunchCfg._fields_}
+
+
 def default_params(dt=1.0, learning_rate=1.0, pre_factor=0.13, post_factor=0.30,
                    pre_trace_decay=0.75, post_trace_decay=0.65):
     """NeuCor's defaults (src/NeuCor.cpp:18-29) with runSpeed = dt."""
@@ -84,6 +88,13 @@ class Network:
         p, s = C.c_uint64(), C.c_uint64()
         check(lib().nc_net_device_size(self._h, C.byref(p), C.byref(s)))
         return p.value, s.value
+
+    def prepare(self):
+        """Builds and keeps the device image (layout, slot records,
+        out-edges), so that each Simulation.add of this network only
+        relocates and uploads it; safe to call from a worker thread."""
+        check(lib().nc_net_prepare(self._h))
+        return self
 
     def close(self):
         if self._h:
@@ -201,23 +212,21 @@ class Simulation:
                 "mask_words_read", "mask_words_written"]
         return dict(zip(keys, (int(x) for x in out)))
 
-    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0, stage_tables=0):
-        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps, stage_tables)
+    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0, stage_tables=0, fanout_waves=0):
+        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps, stage_tables, fanout_waves)
         check(lib().nc_sim_set_launch(self._h, C.byref(cfg)))
 
     def launch(self):
         cfg = LaunchCfg()
         check(lib().nc_sim_get_launch(self._h, C.byref(cfg)))
-        return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
-                "graph_steps": cfg.graph_steps, "stage_tables": cfg.stage_tables}
+        return _cfg_dict(cfg)
 
     def tune(self, steps_per_trial=32):
         """Autotunes the launch shape on the live simulation (advances it by
         2 * 32 * steps_per_trial steps); returns the chosen shape."""
         cfg = LaunchCfg()
         check(lib().nc_sim_tune(self._h, steps_per_trial, C.byref(cfg)))
-        return {"waves_per_block": cfg.waves_per_block, "xcd_swizzle": cfg.xcd_swizzle,
-                "graph_steps": cfg.graph_steps, "stage_tables": cfg.stage_tables}
+        return _cfg_dict(cfg)
 
     def save(self, net_id):
         """The network's dynamic state as bytes (a checkpoint)."""

@@ -51,17 +51,23 @@ def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps
     return out
 
 
-def digest(sim, nid, n, n_syn):
+def read_result(sim, nid, n, n_syn):
+    """A job's final state: potentials, weights, firing counts."""
+    return sim.potential(nid, n), sim.weights(nid, n_syn), sim.firings(nid, n)
+
+
+def summarize(V, w, f):
     """A job's result: its firing counts and a hash of its final potentials
     and weights (bit patterns)."""
-    V = sim.potential(nid, n)
-    w = sim.weights(nid, n_syn)
-    f = sim.firings(nid, n)
     h = hashlib.sha256()
     h.update(V.tobytes())
     h.update(w.tobytes())
     h.update(f.tobytes())
     return {"spikes": int(f.sum()), "hash": h.hexdigest()}
+
+
+def digest(sim, nid, n, n_syn):
+    return summarize(*read_result(sim, nid, n, n_syn))
 
 
 class Job:
@@ -73,8 +79,15 @@ class Job:
         self.net = self.nid = self.t_built = self.t_admit = self.t_done = self.result = None
 
 
+def _finish(job, state, t0):
+    job.result = summarize(*state)
+    job.t_done = time.perf_counter() - t0
+    return job
+
+
 def _build(job):
     net, drive, params, steps = workloads.build_spec(job.spec, n_threads=1)
+    net.prepare()  # the device image, off the serving thread
     job.net, job.drive, job.params, job.steps = net, drive, params, steps
     job.n, job.n_syn = net.n, net.n_syn
     return job
@@ -91,6 +104,7 @@ class Server:
         self.chunk = chunk
         self.max_ready = max_ready  # jobs built or being built but not admitted (bounds host memory)
         self.pool = cf.ThreadPoolExecutor(builders)
+        self.finisher = cf.ThreadPoolExecutor(2)  # results, apart from the builders' queue
 
     @staticmethod
     def capacity_for(free_bytes, fraction=0.5, syn_per_neuron=60):
@@ -100,18 +114,27 @@ class Server:
         n = int(free_bytes * fraction / per_neuron)
         return n, n * syn_per_neuron
 
-    def run(self, jobs, t_limit=None, log=None):
+    def run(self, jobs, t_limit=None, log=None, kernel_every=0):
         """Serves `jobs` (each with an arrival time in seconds from now).
-        Returns the jobs with build/admit/done times and results."""
+        Returns (completed jobs with build/admit/done times and results,
+        summary) where summary holds the unfinished counts and where the
+        loop's wall time went: stepping (launch + wait for the chunk),
+        admission, retirement (result reads and removal) and idling.  With
+        kernel_every > 0 every kernel_every-th step's kernels are timed with
+        HIP events (mean microseconds in the summary)."""
         t0 = time.perf_counter()
+        acc = collections.Counter()
+        if kernel_every:
+            self.sim.set_kernel_timing((1 << 1) | (1 << 3), kernel_every)
         arrivals = sorted(jobs, key=lambda j: j.arrival)
         nxt = 0
         building = set()
         waiting = collections.deque()
         running = {}
+        finishing = []  # results being summarised on the pool
         done = []
         last_log = t0
-        while nxt < len(arrivals) or building or waiting or running:
+        while nxt < len(arrivals) or building or waiting or running or finishing:
             now = time.perf_counter() - t0
             if t_limit is not None and now > t_limit:
                 break
@@ -119,12 +142,16 @@ class Server:
                    and len(building) + len(waiting) < self.max_ready):
                 building.add(self.pool.submit(_build, arrivals[nxt]))
                 nxt += 1
+            for fut in [f for f in finishing if f.done()]:
+                finishing.remove(fut)
+                done.append(fut.result())
             for fut in [f for f in building if f.done()]:
                 building.discard(fut)
                 job = fut.result()
                 job.t_built = time.perf_counter() - t0
                 waiting.append(job)
             # admission in arrival order while the device pools have room
+            ta = time.perf_counter()
             while waiting:
                 job = waiting[0]
                 try:
@@ -144,33 +171,52 @@ class Server:
                     continue
                 waiting.popleft()
                 job.t_admit = time.perf_counter() - t0
+                job.net.close()  # the simulation keeps what it needs
+                job.net = None
                 running[job.nid] = job
+            tb = time.perf_counter()
+            acc["admit_s"] += tb - ta
             if running:
                 self.sim.step(self.chunk)
+                tc = time.perf_counter()
                 self.sim.synchronize()
+                td = time.perf_counter()
+                acc["launch_s"] += tc - tb
+                acc["gpu_wait_s"] += td - tc
+                acc["chunks"] += 1
+                acc["running_sum"] += len(running)
                 for nid in list(running):
                     if self.sim.progress(nid)[1] == 0:
                         job = running.pop(nid)
-                        job.result = digest(self.sim, nid, job.n, job.n_syn)
-                        job.t_done = time.perf_counter() - t0
+                        state = read_result(self.sim, nid, job.n, job.n_syn)
                         self.sim.remove(nid)
-                        job.net.close()
-                        job.net = None
-                        done.append(job)
+                        finishing.append(self.finisher.submit(_finish, job, state, t0))
+                acc["retire_s"] += time.perf_counter() - td
             else:
                 wait = 0.001
                 if nxt < len(arrivals) and not building:
                     wait = max(0.0, min(0.01, arrivals[nxt].arrival - (time.perf_counter() - t0)))
                 time.sleep(wait)
+                acc["idle_s"] += time.perf_counter() - tb
             if log and time.perf_counter() - last_log > 10.0:
                 last_log = time.perf_counter()
                 log("served %d / %d jobs, %d running, %d waiting, %d arrived, %.1fs" %
                     (len(done), len(arrivals), len(running), len(waiting),
                      sum(1 for j in arrivals if j.arrival <= last_log - t0), last_log - t0))
-        return done, {"unfinished": len(arrivals) - len(done), "running": len(running), "waiting": len(waiting)}
+        summary = {"unfinished": len(arrivals) - len(done), "running": len(running), "waiting": len(waiting),
+                   "loop": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in acc.items()}}
+        if acc["chunks"]:
+            summary["loop"]["mean_running"] = acc["running_sum"] / acc["chunks"]
+        if kernel_every:
+            for kind, name in ((1, "k_neuron_us"), (3, "k_fanout_us")):
+                n, ms = self.sim.kernel_timing(kind)
+                summary["loop"][name] = 1000.0 * ms / n if n else None
+            self.sim.set_kernel_timing(0)
+        return done, summary
 
     def close(self):
         self.pool.shutdown(wait=True)
+        self.finisher.shutdown(wait=True)
         self.sim.close()
 
 

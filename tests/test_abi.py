@@ -36,6 +36,8 @@ def test_host_only_calls():
     assert info == {"n": 4, "n_syn": 3, "max_in_degree": 1, "max_delay": 3}
     npos, nslots = net.device_size()
     assert npos == 64 and nslots % 256 == 0 and nslots >= 3
+    # the kept device image is host work and leaves the layout as it was
+    assert net.prepare() is net and net.prepare().device_size() == (npos, nslots)
     assert nc.footprint(1000, 50_000) > 50_000 * 17
     assert nc.lib().nc_last_error() is not None
 
@@ -45,3 +47,4 @@ def test_errors_cross_the_boundary_as_codes():
     assert L.nc_net_info(None, None, None, None, None) == 1  # NC_ERR_ARG
     assert b"null" in L.nc_last_error()
     assert L.nc_sim_step(None, 1) == 1
+    assert L.nc_net_prepare(None) == 1

@@ -66,7 +66,7 @@ def mix_and_ref():
     return nets, budgets, ref
 
 
-@pytest.mark.parametrize("cfg", [(1, 0, 0), (2, 1, 0, 1), (8, 0, 0), (4, 1, 16, 1), (1, 0, 64), (8, 1, 200, 1)])
+@pytest.mark.parametrize("cfg", [(1, 0, 0), (2, 1, 0, 1, 64), (8, 0, 0), (4, 1, 16, 1, 100), (1, 0, 64, 0, 200), (8, 1, 200, 1)])
 def test_launch_shapes_give_identical_results(mix_and_ref, cfg):
     nets, budgets, ref = mix_and_ref
     say("launch shape", cfg)
@@ -110,3 +110,18 @@ def test_graph_chunks_with_host_events():
     assert np.array_equal(a[0][0], b[0][0]) and np.array_equal(a[0][1], b[0][1])
     assert np.array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
     assert np.array_equal(a[2].view(np.uint32), b[2].view(np.uint32))
+
+
+def test_prepared_network_adds_identically():
+    """A network whose device image was built once (nc_net_prepare) and added
+    twice, beside the other networks of a simulation, runs exactly as the same
+    network added without a kept image."""
+    nets, budgets = _mix(seeds=(11, 12), budgets=(700, 500))
+    ref, _ = _run(nets, budgets)
+    prepared = [(n.prepare(), d, p) for n, d, p in nets]
+    got, _ = _run(prepared, budgets)
+    _same(got, ref)
+    twice = prepared + [prepared[0]]
+    got2, _ = _run(twice, budgets + [700])
+    _same(got2[:len(ref)], ref)
+    _same(got2[-1:], ref[:1])
