@@ -33,7 +33,7 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
     the engine's event counters over the timed steps:
       20 B per neuron        in-degree/flags + input group 8, V read+write 8,
                              lastFire read 4
-      36 B per 64 neurons    active-slice record 16, pending count 4,
+      40 B per 64 neurons    active-slice record 16, pending counts 8,
                              fired-lane record write 16
        4 B per mask word     active-slot bit words read (ceil(in-degree/32)
                              per neuron) and 4 B per word written back
@@ -42,12 +42,13 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
       20 B per in-synapse    post-side STDP of a fired neuron: record read 16,
            of a fired neuron w write 4
       12 B per fired neuron  firings read+write 8, lastFire write 4
-       8 B per pending entry scanned (spikes in flight to the slice)"""
+       4 B per pending entry scanned (spikes in flight to the slice in the
+                             list of the step's arrival window)"""
     d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree",
                                                "pending_scanned", "mask_words_read", "mask_words_written")}
-    b = (20.0 * n_neurons + 36.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
+    b = (20.0 * n_neurons + 40.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
          + 16.0 * d["active_slot_steps"] + 16.0 * d["arrivals"] + 20.0 * d["fired_in_degree"] + 12.0 * d["fired"]
-         + 8.0 * d["pending_scanned"])
+         + 4.0 * d["pending_scanned"])
     return b, {"A": d["active_slot_steps"], "R": d["arrivals"], "F": d["fired"], "F_in": d["fired_in_degree"],
                "pending": d["pending_scanned"], "mask_words": d["mask_words_read"]}
 
@@ -301,6 +302,8 @@ def bench_served(args):
                              "identical_results": True},
         "speedup_vs_sequential": (len(done) / t_end) * seq_per_job if (t_end and seq_per_job) else None,
         "serving_loop": rest["loop"],
+        "mean_rate_per_step": (sum(j.result["spikes"] for j in done if "spikes" in j.result) /
+                               max(1, sum(j.n * j.steps for j in done if "spikes" in j.result))),
     }
     print(json.dumps(out), flush=True)
 

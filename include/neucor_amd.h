@@ -177,7 +177,8 @@ nc_status nc_sim_free_bytes(const nc_sim* sim, uint64_t* free_bytes);
 /* Upload a network; its state starts as NeuCor's constructors leave it
  * (V = -70, never fired, no spike in flight).  drive may be NULL.
  * steps_to_run = 0 means "run until told otherwise".  Fails with
- * NC_ERR_CAPACITY when the nc_sim has no room. */
+ * NC_ERR_CAPACITY when the nc_sim has no room, NC_ERR_ARG for an in-degree
+ * above 262140. */
 nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params* params,
                      const nc_drive_spec* drive, uint32_t steps_to_run,
                      int32_t* net_id);

@@ -13,7 +13,8 @@
 //   amask  : 1 bit per slot, set while the slot's spike is in its release
 //            window; words [w][lane] per slice, so a wave reads 256
 //            contiguous bytes per 32 slots of each lane
-//   pend   : per slice, (slot, arrival) of the spikes still in flight
+//   pend0/1: per slice, the spikes still in flight, in two lists by
+//            arrival window ((slot in slice) << 8 | arrival & 255 each)
 //   out-CSR: per position a row of slot indices, u8 delays, target slices
 //            and the source-side copy of each synapse's latch, for the
 //            fan-out.
@@ -88,6 +89,13 @@ struct RangeAlloc {
     uint64_t f = 0;
     for (auto& kv : free_) f += kv.second;
     return f;
+  }
+  // More room at the end (merged with a free tail).
+  void extend(uint64_t c) {
+    if (c <= cap) return;
+    const uint64_t old = cap;
+    cap = c;
+    release(old, c - old);
   }
 };
 
@@ -386,8 +394,8 @@ struct nc_sim {
   uint2* ndw = nullptr;
   uint2* out_row = nullptr;
   nc_edge* edges = nullptr;
-  uint2* pend = nullptr;
-  uint32_t* pend_cnt = nullptr;
+  uint32_t *pend0 = nullptr, *pend1 = nullptr;
+  uint2* pend_cnt = nullptr;
   uint32_t *slice_off = nullptr, *slice_net = nullptr;
   nc_slice_rec* active = nullptr;
   uint32_t *forced_in = nullptr, *forced_bg = nullptr;
@@ -411,8 +419,11 @@ struct nc_sim {
   DevBuf<uint32_t> d_ev_pos;
   DevBuf<uint8_t> d_ev_kind;
 
+  // Drive group table and threshold tables: ranges per network, reused after
+  // removal (freed groups point at the never-live network max_nets).
   std::vector<nc_group> groups;
   std::vector<uint32_t> thr;
+  RangeAlloc a_groups, a_thr;
   std::vector<nc_slice_rec> active_host;
   std::vector<uint32_t> slice_off_host, slice_moff_host;
   bool active_dirty = true, groups_dirty = true;
@@ -497,14 +508,19 @@ nc_step_args base_args(nc_sim* sim) {
   a.r_rel = sim->T.r_rel;
   a.out_row = sim->out_row;
   a.edges = sim->edges;
-  a.pend = sim->pend;
+  a.pend0 = sim->pend0;
+  a.pend1 = sim->pend1;
   a.pend_cnt = sim->pend_cnt;
   a.fired = sim->fired;
   return a;
 }
 
-// The list of slices the step kernels visit: every slice of every live
-// network, networks in id order (each network's slices are contiguous).
+// The list of slices the step kernels visit.  When the live networks' slices
+// span at most 5/4 of their number, the list is that whole span in slice
+// order, holes included (records of the never-live network max_nets), so
+// that record i is slice q0 + i and the kernels' per-neuron loads need not
+// wait for the record; otherwise every slice of every live network,
+// networks in id order (each network's slices are contiguous).
 nc_status rebuild_active(nc_sim* sim) {
   if (!sim->active_dirty) return NC_OK;
   if (sim->upload_pending) {
@@ -513,6 +529,34 @@ nc_status rebuild_active(nc_sim* sim) {
   }
   NC_HIP(hipStreamSynchronize(sim->stream));
   sim->active_host.clear();
+  uint32_t qmin = UINT32_MAX, qmax = 0;
+  uint64_t nlive = 0;
+  for (const NetRec& r : sim->nets) {
+    if (!r.live || r.nq == 0) continue;
+    qmin = std::min(qmin, r.q0);
+    qmax = std::max(qmax, r.q0 + r.nq);
+    nlive += r.nq;
+  }
+  if (nlive && (uint64_t)(qmax - qmin) * 4 <= nlive * 5) {
+    for (uint32_t q = qmin; q < qmax; ++q) {
+      const uint32_t id = sim->slice_net_host[q];
+      if (id != NC_SLICE_NONE && sim->nets[id].live)
+        sim->active_host.push_back(nc_slice_rec{q, sim->slice_off_host[q], id, sim->slice_moff_host[q]});
+      else
+        sim->active_host.push_back(nc_slice_rec{q, 0u, sim->max_nets, 0u});
+    }
+    for (NetRec& r : sim->nets) {
+      if (!r.live) continue;
+      r.act0 = r.nq ? r.q0 - qmin : 0u;
+      r.act_valid = true;
+    }
+    sim->act_identity = true;
+    sim->act_q0 = qmin;
+    NC_HIP(hipMemcpy(sim->active, sim->active_host.data(), sim->active_host.size() * sizeof(nc_slice_rec),
+                     hipMemcpyHostToDevice));
+    sim->active_dirty = false;
+    return NC_OK;
+  }
   for (size_t i = 0; i < sim->nets.size(); ++i) {
     NetRec& r = sim->nets[i];
     if (!r.live) continue;
@@ -569,7 +613,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   const uint64_t slots = (n_syn + n_syn / 4 + P * 4 + 255) / 256 * 256;  // padding allowance
   const uint64_t mwords = slots / 32 + 64 * Q;
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
-           + Q * (4 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
+           + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
            + (n_syn + 64) * sizeof(nc_edge)                // out-edges
            + (Q + 2) * NC_NCTR * 8;                        // counters
@@ -642,8 +686,9 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMalloc(&sim->nid, P * 4), "hipMalloc nid");
     H(hipMalloc(&sim->out_row, P * sizeof(uint2)), "hipMalloc out_row");
     H(hipMalloc(&sim->edges, sim->max_out * sizeof(nc_edge)), "hipMalloc edges");
-    H(hipMalloc(&sim->pend, sim->max_slots * sizeof(uint2)), "hipMalloc pend");
-    H(hipMalloc(&sim->pend_cnt, Q * 4), "hipMalloc pend_cnt");
+    H(hipMalloc(&sim->pend0, sim->max_slots * 4), "hipMalloc pend0");
+    H(hipMalloc(&sim->pend1, sim->max_slots * 4), "hipMalloc pend1");
+    H(hipMalloc(&sim->pend_cnt, Q * sizeof(uint2)), "hipMalloc pend_cnt");
     H(hipMalloc(&sim->slice_off, Q * 4), "hipMalloc slice_off");
     H(hipMalloc(&sim->slice_net, Q * 4), "hipMalloc slice_net");
     H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
@@ -656,7 +701,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     sim->ctr_rows = Q + 2;
     H(hipMalloc(&sim->d_chunk_off, 256), "hipMalloc chunk_off");
     H(hipMalloc(&sim->ctr, sim->ctr_rows * NC_NCTR * 8), "hipMalloc ctr");
-    H(hipMalloc(&sim->d_nets, sim->max_nets * sizeof(nc_netdesc)), "hipMalloc nets");
+    // one more, never written: the (never live) network of hole records
+    H(hipMalloc(&sim->d_nets, (sim->max_nets + 1ull) * sizeof(nc_netdesc)), "hipMalloc nets");
   }
   if (st == NC_OK) {
     H(hipMemsetAsync(sim->amask, 0, sim->max_mwords * 4, sim->stream), "memset");
@@ -665,9 +711,11 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->fired, 0, Q * sizeof(uint4), sim->stream), "memset");
+    H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
     H(hipMemsetAsync(sim->ctr, 0, sim->ctr_rows * NC_NCTR * 8, sim->stream), "memset");
+    H(hipMemsetAsync(sim->d_nets, 0, (sim->max_nets + 1ull) * sizeof(nc_netdesc), sim->stream), "memset");
     std::vector<uint32_t> none(Q, NC_SLICE_NONE);
     sim->slice_net_host = none;
     sim->slice_off_host.assign(Q, 0u);
@@ -693,7 +741,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
-                  sim->nid, sim->out_row, sim->edges, sim->pend, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p};
@@ -744,7 +792,8 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     return fail(NC_ERR_ARG, "all networks of one nc_sim share dt and the trace decays");
   }
   if ((uint32_t)sim->T.r_rel >= 250u) return fail(NC_ERR_ARG, "dt too fine: the release window must be < 250 steps");
-  if (net->max_in_degree > NC_DEG_MASK) return fail(NC_ERR_ARG, "in-degree too large");
+  // a pending entry holds a slot's offset inside its slice in 24 bits
+  if (net->max_in_degree > NC_MAX_IN_DEGREE) return fail(NC_ERR_ARG, "in-degree too large (at most 262140)");
 
   // Free network id.
   int32_t id = -1;
@@ -829,29 +878,48 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     }
     r.has_drive = true;
     r.drive = *drive;
-    r.grp0 = (uint32_t)sim->groups.size();
-    r.ngrp = sel.n_groups_total;
-    r.thr0 = (uint32_t)sim->thr.size();
     const uint32_t T = drive->period ? drive->period : 1;
     const uint32_t ncorr = drive->n_correlated > 0 ? drive->n_groups : 0;
-    for (uint32_t g = 0; g < sel.n_groups_total; ++g) {
-      nc_group G{};
-      G.tbl_off = (uint32_t)sim->thr.size();
-      G.period = T;
-      G.local = g;
-      G.net = (uint32_t)id;
-      const double amp = g < ncorr ? drive->amp_corr : drive->amp_ind;
+    // Room for this network's groups and its two threshold tables (one per
+    // amplitude: correlated groups, independent inputs).
+    uint64_t g0 = 0, t0 = 0;
+    const nc_group dead{0u, 1u, 0u, sim->max_nets};
+    bool grew = false;
+    auto room = [&grew](RangeAlloc& A, auto& host, uint64_t n, auto fill, uint64_t* off) {
+      if (A.alloc(n, off)) return;
+      const uint64_t c = std::max<uint64_t>({A.cap * 2, A.cap + n, 1024});
+      A.extend(c);
+      host.resize(c, fill);
+      grew = true;
+      A.alloc(n, off);
+    };
+    room(sim->a_groups, sim->groups, sel.n_groups_total, dead, &g0);
+    room(sim->a_thr, sim->thr, 2ull * T, 0u, &t0);
+    r.grp0 = (uint32_t)g0;
+    r.ngrp = sel.n_groups_total;
+    r.thr0 = (uint32_t)t0;
+    r.nthr = 2 * T;
+    for (uint32_t tb = 0; tb < 2; ++tb) {
+      const double amp = tb == 0 ? drive->amp_corr : drive->amp_ind;
       for (uint32_t k = 0; k < T; ++k) {
         double pr = (double)drive->p0 * (1.0 + amp * sin(2.0 * M_PI * (double)k / (double)T));
         uint32_t th;
         if (!(pr > 0.0)) th = 0;
         else if (pr >= 1.0) th = 0xFFFFFFFFu;
         else th = (uint32_t)floor(pr * 4294967296.0);
-        sim->thr.push_back(th);
+        sim->thr[t0 + (uint64_t)tb * T + k] = th;
       }
-      sim->groups.push_back(G);
     }
-    r.nthr = (uint32_t)sim->thr.size() - r.thr0;
+    for (uint32_t g = 0; g < sel.n_groups_total; ++g)
+      sim->groups[g0 + g] = nc_group{(uint32_t)t0 + (g < ncorr ? 0u : T), T, g, (uint32_t)id};
+    if (grew || sim->d_groups.n < sim->groups.size() || sim->d_thr.n < sim->thr.size()) {
+      sim->groups_dirty = true;  // whole tables, at the next step
+    } else if (r.ngrp) {
+      NC_HIP(hipMemcpyAsync(sim->d_groups.p + g0, sim->groups.data() + g0, r.ngrp * sizeof(nc_group),
+                            hipMemcpyHostToDevice, sim->stream));
+      NC_HIP(hipMemcpyAsync(sim->d_thr.p + t0, sim->thr.data() + t0, r.nthr * 4ull, hipMemcpyHostToDevice,
+                            sim->stream));
+    }
     for (size_t k = 0; k < sel.ids.size(); ++k) {
       uint32_t pl = M.pos_of_id[sel.ids[k]];
       ndwv[pl].x |= NC_DEG_INPUT;
@@ -867,7 +935,6 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       nd.bg_lo = (uint32_t)bs;
       nd.bg_hi = (uint32_t)(bs >> 32);
     }
-    sim->groups_dirty = true;
   }
   std::vector<float> V0(npos, NC_BASELEVEL);
   std::vector<int32_t> lf0(npos, NC_NEVER);
@@ -889,7 +956,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
                        (uint32_t)q0);
     NC_HIP(hipGetLastError());
   }
-  NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * 4, st));
+  NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * sizeof(uint2), st));
   NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
@@ -918,18 +985,19 @@ extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) {
   sim->a_slots.release(r->slot0 / 256, r->nslots / 256);
   sim->a_out.release(r->out0, r->S);
   sim->a_masks.release(r->mask0 / 64, r->nmwords / 64);
-  // Drive groups stay allocated in the host vectors (append-only); they are
-  // compacted when no network uses them.
+  if (r->has_drive) {
+    // the groups go back to the table, pointing at the never-live network
+    const nc_group dead{0u, 1u, 0u, sim->max_nets};
+    for (uint32_t g = 0; g < r->ngrp; ++g) sim->groups[r->grp0 + g] = dead;
+    if (r->ngrp && !sim->groups_dirty)
+      NC_HIP(hipMemcpy(sim->d_groups.p + r->grp0, sim->groups.data() + r->grp0, r->ngrp * sizeof(nc_group),
+                       hipMemcpyHostToDevice));
+    sim->a_groups.release(r->grp0, r->ngrp);
+    sim->a_thr.release(r->thr0, r->nthr);
+  }
   r->live = false;
   r->maps.reset();
   sim->active_dirty = true;
-  bool any = false;
-  for (auto& x : sim->nets) any |= x.live;
-  if (!any) {
-    sim->groups.clear();
-    sim->thr.clear();
-    sim->groups_dirty = true;
-  }
   return NC_OK;
 }
 
@@ -1523,14 +1591,23 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipMemcpyAsync(sim->asum + pos0, asum.data(), npos * 4, hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->amask + r->mask0, masks.data(), r->nmwords * 4, hipMemcpyHostToDevice, sq));
-  std::vector<uint32_t> cnt(r->nq);
+  // each slice's spikes in flight into the pending list of their arrival
+  // window, entries relative to the slice's first slot
+  std::vector<uint2> cnt2(r->nq);
+  std::vector<std::vector<uint32_t>> lists(2 * (size_t)r->nq);
   for (uint32_t q = 0; q < r->nq; ++q) {
-    cnt[q] = (uint32_t)pend[q].size();
-    if (cnt[q])
-      NC_HIP(hipMemcpyAsync(sim->pend + sim->slice_off_host[r->q0 + q], pend[q].data(), cnt[q] * sizeof(uint2),
-                            hipMemcpyHostToDevice, sq));
+    const uint32_t first = sim->slice_off_host[r->q0 + q];
+    for (const uint2& e : pend[q])
+      lists[2 * q + ((e.y >> NC_PEND_LOG_W) & 1u)].push_back(((e.x - first) << 8) | (e.y & 255u));
+    cnt2[q] = make_uint2((uint32_t)lists[2 * q].size(), (uint32_t)lists[2 * q + 1].size());
+    for (uint32_t k = 0; k < 2; ++k) {
+      const std::vector<uint32_t>& l = lists[2 * q + k];
+      if (!l.empty())
+        NC_HIP(hipMemcpyAsync((k ? sim->pend1 : sim->pend0) + first, l.data(), l.size() * 4, hipMemcpyHostToDevice,
+                              sq));
+    }
   }
-  NC_HIP(hipMemcpyAsync(sim->pend_cnt + r->q0, cnt.data(), r->nq * 4, hipMemcpyHostToDevice, sq));
+  NC_HIP(hipMemcpyAsync(sim->pend_cnt + r->q0, cnt2.data(), r->nq * sizeof(uint2), hipMemcpyHostToDevice, sq));
   std::vector<nc_edge> ed(r->S);
   if (r->S) NC_HIP(hipMemcpy(ed.data(), sim->edges + r->out0, r->S * sizeof(nc_edge), hipMemcpyDeviceToHost));
   for (uint64_t k = 0; k < r->S; ++k) ed[r->maps->edge_of_syn[k]].arr = edge_arr[k];

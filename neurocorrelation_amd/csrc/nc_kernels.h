@@ -8,10 +8,11 @@
 //             charge_passive, threshold / forced fire, AP waveform; then the
 //             wave applies post-side STDP to the incoming synapses of each
 //             of its fired lanes, and turns the slice's spikes arriving at s
-//             (pending list) into active-slot bits
+//             (the pending list of s's window) into active-slot bits
 //   k_fanout  one wave per slice with fired lanes: Synapse::fire on their
 //             outgoing synapses (source-order latch copy, out->in slot map),
 //             appending accepted spikes to the target slice's pending list
+//             of their arrival's window
 // The kernel boundary orders the phases.  Every synapse slot has one writer
 // per phase (its target's wave in k_neuron, its source's wave in k_fanout);
 // the only atomics are the pending-list append (per target slice) and the
@@ -28,6 +29,13 @@
 #define NC_LDS_TAB_BYTES 32768  // trace tables up to this size are staged in LDS
 #define NC_WAP_K 8       // AP-table entries passed as kernel arguments
 #define NC_MW 4          // active-mask words (32 slots each) a lane holds per pass
+// Spikes in flight to a slice sit in one of two pending lists by arrival
+// window: list (arrival >> NC_PEND_LOG_W) & 1, each at the slice's slot range
+// of its own array (at most one spike in flight per slot).  An entry is
+// (slot - first slot of the slice) << 8 | (arrival & 255): a list scanned at
+// step s holds no arrival before s nor past s + 255 (delays are 1..255).
+#define NC_PEND_LOG_W 4
+#define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay below 2^24 (64 lanes x 65535 chunks of 4)
 #define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
 
 // Per-network descriptor, rewritten by the host before each batch of steps.
@@ -55,7 +63,7 @@ struct nc_edge {
   uint32_t tq_d;   // target slice << 8 | delay in steps
   int32_t arr;     // arrival step of the last accepted spike (NC_NEVER: none);
                    // the latch holds until arrival + release window
-  uint32_t tbase;  // first pending-list entry of the target slice
+  uint32_t tbase;  // first slot of the target slice (where its pending lists start)
 };
 
 struct nc_group {
@@ -108,8 +116,9 @@ struct nc_step_args {
   uint32_t xcd_swizzle;       // blocks of one XCD take one contiguous range of slices
   const uint2* out_row;     // per position: first out-edge, out-degree
   nc_edge* edges;           // out-edges, rows in ascending target id
-  uint2* pend;             // per slice, at its slot range: (slot, arrival step) of spikes in flight
-  uint32_t* pend_cnt;      // per slice: entries in its pending list
+  uint32_t* pend0;         // pending list 0 of every slice, at its slot range
+  uint32_t* pend1;         // pending list 1
+  uint2* pend_cnt;         // per slice: entries in pending list 0, in list 1
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
   uint32_t n_ev;
@@ -252,21 +261,24 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   uint2 nw = a.ndw[pos];
   float V = a.V[pos];
   int32_t lf = a.last_fire[pos];
-  uint32_t pend_n = a.pend_cnt[q];
+  const uint2 pc = a.pend_cnt[q];
   uint32_t sm = a.asum[pos];
   const nc_netdesc nd = a.nets[rec.net];
   const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;  // step index inside this call
   const bool live = wave_in && ci < nd.remaining;
+  const int32_t s = nd.base_step + ci + 1;
+  // The pending list of s's arrival window.
+  const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
+  uint32_t pend_n = plist ? pc.y : pc.x;
   if (!live) {
     nw = make_uint2(0u, 0u);
     pend_n = 0u;
     sm = 0u;
   }
+  uint32_t* const pl = (plist ? a.pend1 : a.pend0) + rec.slot_off;
   // First 64 pending entries, fetched now and used at the end of the step.
-  uint2* const pl = a.pend + rec.slot_off;
-  uint2 pe0 = make_uint2(0u, 0u);
+  uint32_t pe0 = 0u;
   if (lane < pend_n) pe0 = pl[lane];
-  const int32_t s = nd.base_step + ci + 1;
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
@@ -440,10 +452,10 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
     uint32_t kept = 0;
     for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
       const uint32_t t = t0 + lane;
-      uint2 e = pe0;
+      uint32_t e = pe0;
       if (t0 && t < pend_n) e = pl[t];
-      const bool arrive = t < pend_n && (int32_t)e.y == s;
-      const uint32_t rel = e.x - rec.slot_off;
+      const bool arrive = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
+      const uint32_t rel = e >> 8;
       const uint32_t tl = (rel & 255u) >> 2;
       const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
       const uint32_t Gt = (uint32_t)__shfl((int)G, (int)tl);
@@ -459,7 +471,7 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
       if (keep && to != t) pl[to] = e;
       kept += (uint32_t)__builtin_popcountll(km);
     }
-    if (lane == 0) a.pend_cnt[q] = kept;
+    if (lane == 0) reinterpret_cast<uint32_t*>(&a.pend_cnt[q])[plist] = kept;
   }
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
   const uint32_t sm_new = sh_sum[wl][lane];
@@ -498,8 +510,10 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   a.slots[ed.j].depol = a.depol_k * r.w;
   a.slots[ed.j].arr_inh = arr * 2 + (r.arr_inh & 1);
   a.edges[e].arr = arr;
-  const uint32_t at = atomicAdd(&a.pend_cnt[ed.tq_d >> 8], 1u);
-  a.pend[ed.tbase + at] = make_uint2(ed.j, (uint32_t)arr);
+  const uint32_t tq = ed.tq_d >> 8;
+  const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
+  const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
+  (plist ? a.pend1 : a.pend0)[ed.tbase + at] = ((ed.j - ed.tbase) << 8) | ((uint32_t)arr & 255u);
   ++n_acc;
 }
 
