@@ -264,7 +264,7 @@ def bench_served(args):
     free = probe.free_bytes()
     probe.close()
     max_n, max_s = SV.Server.capacity_for(free, fraction=0.5)
-    server = SV.Server(max_n, max_s, max_nets=4096, chunk=args.chunk)
+    server = SV.Server(max_n, max_s, max_nets=4096, chunk=args.chunk, lanes=args.lanes)
     print("[bench] served: %d jobs over %.0fs, device room %d neurons / %d synapses" % (n_jobs, args.window, max_n,
           max_s), file=sys.stderr, flush=True)
     done, rest = server.run(jobs, t_limit=args.wall_cap, kernel_every=args.kernel_every,
@@ -289,7 +289,7 @@ def bench_served(args):
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "served", "rate_per_s": args.rate, "window_s": args.window, "jobs": n_jobs,
                    "neurons": [300, args.max_neurons], "synapses_per_neuron": [20, 100],
-                   "job_steps": [args.min_job_steps, args.max_job_steps], "chunk": args.chunk,
+                   "job_steps": [args.min_job_steps, args.max_job_steps], "chunk": args.chunk, "lanes": args.lanes,
                    "weights": {"w_min": weights[0], "w_max": weights[1], "inhib_frac": weights[2]},
                    "wall_cap_s": args.wall_cap},
         "completed": len(done), "unfinished": rest["unfinished"],
@@ -330,6 +330,8 @@ def main():
     ap.add_argument("--min-job-steps", type=int, default=1_000, help="served: shortest job")
     ap.add_argument("--max-job-steps", type=int, default=50_000, help="served: longest job")
     ap.add_argument("--wall-cap", type=float, default=900.0, help="served: stop serving after this many seconds")
+    ap.add_argument("--lanes", type=int, default=2,
+                    help="served: simulations the GPU alternates between (host work of one overlaps the other's chunk)")
     ap.add_argument("--kernel-every", type=int, default=0,
                     help="served: time every K-th step's kernels with HIP events (0: none)")
     ap.add_argument("--runaway", action="store_true",

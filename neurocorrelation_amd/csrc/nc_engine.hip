@@ -166,8 +166,10 @@ struct NetImage {
   uint32_t nq = 0;
   uint64_t S = 0, nslots = 0, nmwords = 0, mwords_step = 0;
   std::vector<uint32_t> slice_off, slice_moff;  // per slice: first slot, first mask word
-  std::vector<nc_slot> slots;                   // nslots records
-  std::vector<nc_edge> edges;                   // S out-edges by source position, rows in ascending target id
+  std::vector<float> slot_w;                    // nslots weights (0 on padding slots)
+  std::vector<uint32_t> slot_inh;               // one sign bit per slot (1: inhibitory class)
+  std::vector<uint2> edge_jq;                   // S out-edges by source position, rows in ascending target
+                                                // id: (target slot, target slice << 8 | delay)
   std::vector<uint2> out_row;                   // per position: (first out-edge, count)
   std::vector<uint32_t> degw;                   // per position: in-degree | NC_DEG_VALID (0: padding)
   std::vector<uint32_t> nid;                    // per position: neuron id (~0u: padding)
@@ -225,7 +227,8 @@ std::shared_ptr<const NetImage> build_image(const nc_net& net) {
   M.digest = h;
   M.in_ptr = net.in_ptr;
   // Slot records, in layout order.
-  I.slots.assign(L.nslots, nc_slot{0.0f, 0.0f, NC_NEVER, 2 * NC_NEVER});
+  I.slot_w.assign(L.nslots, 0.0f);
+  I.slot_inh.assign((L.nslots + 31) / 32, 0u);
   M.slot_of_syn.resize(S);
   M.edge_of_syn.assign(S, 0u);
   for (uint32_t id = 0; id < net.n; ++id) {
@@ -235,9 +238,8 @@ std::shared_ptr<const NetImage> build_image(const nc_net& net) {
     for (int64_t k = 0; k < e - b; ++k) {
       const uint32_t j = I.slice_off[q] + (uint32_t)((k >> 2) * 256 + l * 4 + (k & 3));
       M.slot_of_syn[b + k] = j;
-      nc_slot& x = I.slots[j];
-      x.w = net.w[b + k];
-      x.arr_inh = 2 * NC_NEVER + (net.inh[b + k] ? 1 : 0);
+      I.slot_w[j] = net.w[b + k];
+      if (net.inh[b + k]) I.slot_inh[j >> 5] |= 1u << (j & 31);
     }
   }
   // Out-CSR by source position; rows in ascending target id.
@@ -250,14 +252,14 @@ std::shared_ptr<const NetImage> build_image(const nc_net& net) {
       o += outcnt[pl];
     }
   }
-  I.edges.resize(S);
+  I.edge_jq.resize(S);
   {
     std::vector<uint32_t> fillp(outbeg);
     for (uint32_t tg = 0; tg < net.n; ++tg) {
       const uint32_t tq = L.pos_of_id[tg] / 64;
       for (int64_t k = net.in_ptr[tg]; k < net.in_ptr[tg + 1]; ++k) {
         const uint32_t e = fillp[L.pos_of_id[net.in_src[k]]]++;
-        I.edges[e] = nc_edge{M.slot_of_syn[k], (tq << 8) | net.delay[k], NC_NEVER, I.slice_off[tq]};
+        I.edge_jq[e] = make_uint2(M.slot_of_syn[k], (tq << 8) | net.delay[k]);
         M.edge_of_syn[k] = e;
       }
     }
@@ -415,6 +417,8 @@ struct nc_sim {
   float *t_pre = nullptr, *t_post = nullptr, *w_ap = nullptr;
   DevBuf<nc_group> d_groups;
   DevBuf<uint32_t> d_gfire;
+  DevBuf<float> d_wbuf;       // packed weights of one network, for host reads
+  DevBuf<uint32_t> d_stage;   // a network's compact upload, expanded in place by the placing kernels
   DevBuf<uint32_t> d_thr;
   DevBuf<uint32_t> d_ev_pos;
   DevBuf<uint8_t> d_ev_kind;
@@ -744,7 +748,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
                   sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
-                  sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p};
+                  sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p, sim->d_wbuf.p, sim->d_stage.p};
   for (void* p : ptrs)
     if (p) (void)hipFree(p);
   for (hipEvent_t e : sim->ev_pool) (void)hipEventDestroy(e);
@@ -936,28 +940,40 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
       nd.bg_hi = (uint32_t)(bs >> 32);
     }
   }
-  std::vector<float> V0(npos, NC_BASELEVEL);
-  std::vector<int32_t> lf0(npos, NC_NEVER);
-  std::vector<uint32_t> zero(npos, 0), snet(I.nq, (uint32_t)id);
+  std::vector<uint32_t> snet(I.nq, (uint32_t)id);
   hipStream_t st = sim->stream;
-  NC_HIP(hipMemcpyAsync(sim->slots + sl0, I.slots.data(), I.nslots * sizeof(nc_slot), hipMemcpyHostToDevice, st));
+  // The compact image (4 bytes and a bit per slot, 8 bytes per edge) goes up
+  // to a staging buffer; the placing kernels expand it into the records.
+  const uint64_t n_inh = I.slot_inh.size(), o_jq = (I.nslots + n_inh + 1) & ~1ull;
+  {
+    nc_status sg = grow(&sim->d_stage, o_jq + 2 * S + 2, st);
+    if (sg != NC_OK) return sg;
+  }
+  uint32_t* const stg = sim->d_stage.p;
+  auto blocks = [](uint64_t n) { return dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 4096)); };
+  NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), I.nq * 4, hipMemcpyHostToDevice, st));
+  if (I.nslots) {
+    NC_HIP(hipMemcpyAsync(stg, I.slot_w.data(), I.nslots * 4, hipMemcpyHostToDevice, st));
+    NC_HIP(hipMemcpyAsync(stg + I.nslots, I.slot_inh.data(), n_inh * 4, hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_place_slots, blocks(I.nslots), dim3(256), 0, st, sim->slots + sl0, (const float*)stg,
+                       (const uint32_t*)(stg + I.nslots), (uint64_t)I.nslots);
+    NC_HIP(hipGetLastError());
+  }
+  if (S) {
+    NC_HIP(hipMemcpyAsync(stg + o_jq, I.edge_jq.data(), S * sizeof(uint2), hipMemcpyHostToDevice, st));
+    hipLaunchKernelGGL(k_place_edges, blocks(S), dim3(256), 0, st, sim->edges + o0, (const uint2*)(stg + o_jq), S,
+                       (uint32_t)sl0, (uint32_t)q0, (const uint32_t*)sim->slice_off);
+    NC_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_place_neurons, blocks(npos), dim3(256), 0, st, sim->V + pos0, sim->last_fire + pos0,
+                     sim->firings + pos0, (uint64_t)npos);
+  NC_HIP(hipGetLastError());
   NC_HIP(hipMemsetAsync(sim->amask + mk0, 0, I.nmwords * 4, st));
   NC_HIP(hipMemsetAsync(sim->asum + pos0, 0, npos * 4, st));
-  NC_HIP(hipMemcpyAsync(sim->V + pos0, V0.data(), npos * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->last_fire + pos0, lf0.data(), npos * 4, hipMemcpyHostToDevice, st));
-  NC_HIP(hipMemcpyAsync(sim->firings + pos0, zero.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->ndw + pos0, ndwv.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->nid + pos0, I.nid.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->out_row + pos0, orow.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
-  if (S) {
-    NC_HIP(hipMemcpyAsync(sim->edges + o0, I.edges.data(), S * sizeof(nc_edge), hipMemcpyHostToDevice, st));
-    const uint32_t rb = (uint32_t)std::min<uint64_t>((S + 255) / 256, 4096);
-    hipLaunchKernelGGL(k_relocate_edges, dim3(rb), dim3(256), 0, st, sim->edges + o0, S, (uint32_t)sl0,
-                       (uint32_t)q0);
-    NC_HIP(hipGetLastError());
-  }
   NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * sizeof(uint2), st));
-  NC_HIP(hipMemcpyAsync(sim->slice_off + q0, slice_off.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
   NC_HIP(hipStreamSynchronize(st));
@@ -1333,10 +1349,19 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   if (st != NC_OK) return st;
   if (!out) return fail(NC_ERR_ARG, "null argument");
   if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
-  std::vector<nc_slot> hs;
-  st = read_slots(sim, *r, &hs);
+  NC_HIP(hipSetDevice(sim->device));
+  st = flush_net(sim, *r);
   if (st != NC_OK) return st;
-  for (uint64_t k = 0; k < n; ++k) out[k] = hs[r->maps->slot_of_syn[k]].w;
+  if (r->nslots == 0) return NC_OK;
+  st = grow(&sim->d_wbuf, r->nslots, sim->stream);
+  if (st != NC_OK) return st;
+  const uint32_t gb = (uint32_t)std::min<uint64_t>((r->nslots + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_gather_w, dim3(gb), dim3(256), 0, sim->stream, sim->slots + r->slot0, r->nslots, sim->d_wbuf.p);
+  NC_HIP(hipGetLastError());
+  std::vector<float> w(r->nslots);
+  NC_HIP(hipMemcpyAsync(w.data(), sim->d_wbuf.p, r->nslots * 4, hipMemcpyDeviceToHost, sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  for (uint64_t k = 0; k < n; ++k) out[k] = w[r->maps->slot_of_syn[k]];
   return NC_OK;
 }
 

@@ -190,17 +190,42 @@ __global__ void k_set_u32(uint32_t* p, uint32_t n, uint32_t v) {
   if (threadIdx.x < n) p[threadIdx.x] = v;
 }
 
-// Moves a network's out-edges from network-local to simulation indices:
-// target slot and slice base by the network's first slot, target slice by
-// its first slice (tq_d = slice << 8 | delay).
-__global__ __launch_bounds__(256) void k_relocate_edges(nc_edge* e, uint64_t n, uint32_t slot0, uint32_t q0) {
+// The weights of a run of slot records, packed 4 bytes per slot (host reads
+// move a quarter of the records' bytes).
+__global__ __launch_bounds__(256) void k_gather_w(const nc_slot* slots, uint64_t n, float* out) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) out[i] = slots[i].w;
+}
+
+// A network's out-edges from their uploaded network-local form (target slot,
+// target slice << 8 | delay): target slot moved by the network's first slot,
+// target slice by its first slice, the slice's first slot from slice_off
+// (already in place), no spike latched.
+__global__ __launch_bounds__(256) void k_place_edges(nc_edge* e, const uint2* jq, uint64_t n, uint32_t slot0,
+                                                     uint32_t q0, const uint32_t* slice_off) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
-    nc_edge x = e[i];
-    x.j += slot0;
-    x.tq_d += q0 << 8;
-    x.tbase += slot0;
-    e[i] = x;
+    const uint2 x = jq[i];
+    const uint32_t tq_d = x.y + (q0 << 8);
+    e[i] = nc_edge{x.x + slot0, tq_d, NC_NEVER, slice_off[tq_d >> 8]};
+  }
+}
+
+// A network's slot records from their uploaded weights and sign bits (one bit
+// per slot): no spike latched or arrived yet.
+__global__ __launch_bounds__(256) void k_place_slots(nc_slot* s, const float* w, const uint32_t* inh, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    s[i] = nc_slot{w[i], 0.0f, NC_NEVER, 2 * NC_NEVER + (int32_t)((inh[i >> 5] >> (i & 31)) & 1u)};
+}
+
+// Fresh neuron state: potential at the base level, never fired, no firings.
+__global__ __launch_bounds__(256) void k_place_neurons(float* V, int32_t* last_fire, uint32_t* firings, uint64_t n) {
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    V[i] = NC_BASELEVEL;
+    last_fire[i] = NC_NEVER;
+    firings[i] = 0u;
   }
 }
 

@@ -93,18 +93,34 @@ def _build(job):
     return job
 
 
-class Server:
-    """One simulation holding up to `max_nets` jobs within `max_neurons` /
-    `max_synapses` of device room; `chunk` steps between admission and
-    retirement checks."""
+class _Lane:
+    """One simulation of the server and the jobs it runs."""
 
-    def __init__(self, max_neurons, max_synapses, max_nets=4096, chunk=200, builders=8, device=0, max_ready=256):
-        slots = max_synapses + max_synapses // 4 + (max_neurons + 64 * max_nets) * 4
-        self.sim = Simulation(max_neurons, max_synapses, max_nets=max_nets, device=device, max_slots=slots)
+    def __init__(self, sim):
+        self.sim = sim
+        self.running = {}  # net id -> job
+
+
+class Server:
+    """Simulations ("lanes", each with its own stream) sharing `max_neurons` /
+    `max_synapses` of device room and up to `max_nets` jobs; `chunk` steps
+    between admission and retirement checks.  With two lanes, the GPU runs
+    one lane's chunk while the host retires and admits the other's jobs."""
+
+    def __init__(self, max_neurons, max_synapses, max_nets=4096, chunk=200, builders=8, device=0, max_ready=256,
+                 lanes=2, sim_factory=Simulation):
+        n, syn, nets = max_neurons // lanes, max_synapses // lanes, max(1, max_nets // lanes)
+        slots = syn + syn // 4 + (n + 64 * nets) * 4
+        self.lanes = [_Lane(sim_factory(n, syn, max_nets=nets, device=device, max_slots=slots))
+                      for _ in range(lanes)]
         self.chunk = chunk
         self.max_ready = max_ready  # jobs built or being built but not admitted (bounds host memory)
         self.pool = cf.ThreadPoolExecutor(builders)
         self.finisher = cf.ThreadPoolExecutor(2)  # results, apart from the builders' queue
+
+    @property
+    def sim(self):
+        return self.lanes[0].sim
 
     @staticmethod
     def capacity_for(free_bytes, fraction=0.5, syn_per_neuron=60):
@@ -118,81 +134,102 @@ class Server:
         """Serves `jobs` (each with an arrival time in seconds from now).
         Returns (completed jobs with build/admit/done times and results,
         summary) where summary holds the unfinished counts and where the
-        loop's wall time went: stepping (launch + wait for the chunk),
+        loop's wall time went: launching chunks, waiting for the GPU,
         admission, retirement (result reads and removal) and idling.  With
         kernel_every > 0 every kernel_every-th step's kernels are timed with
         HIP events (mean microseconds in the summary)."""
         t0 = time.perf_counter()
         acc = collections.Counter()
         if kernel_every:
-            self.sim.set_kernel_timing((1 << 1) | (1 << 3), kernel_every)
+            for ln in self.lanes:
+                ln.sim.set_kernel_timing((1 << 1) | (1 << 3), kernel_every)
         arrivals = sorted(jobs, key=lambda j: j.arrival)
         nxt = 0
         building = set()
         waiting = collections.deque()
-        running = {}
         finishing = []  # results being summarised on the pool
         done = []
+        in_flight = None  # the lane whose chunk the GPU is running
         last_log = t0
-        while nxt < len(arrivals) or building or waiting or running or finishing:
+
+        def n_running():
+            return sum(len(ln.running) for ln in self.lanes)
+
+        while nxt < len(arrivals) or building or waiting or finishing or n_running():
             now = time.perf_counter() - t0
             if t_limit is not None and now > t_limit:
                 break
-            while (nxt < len(arrivals) and arrivals[nxt].arrival <= now
-                   and len(building) + len(waiting) < self.max_ready):
-                building.add(self.pool.submit(_build, arrivals[nxt]))
-                nxt += 1
-            for fut in [f for f in finishing if f.done()]:
-                finishing.remove(fut)
-                done.append(fut.result())
-            for fut in [f for f in building if f.done()]:
-                building.discard(fut)
-                job = fut.result()
-                job.t_built = time.perf_counter() - t0
-                waiting.append(job)
-            # admission in arrival order while the device pools have room
-            ta = time.perf_counter()
-            while waiting:
-                job = waiting[0]
-                try:
-                    job.nid = self.sim.add(job.net, job.params, job.drive, steps=job.steps)
-                except NCError as e:
-                    if e.code != 5:  # anything but NC_ERR_CAPACITY
-                        raise
-                    if running:  # wait for room
-                        break
-                    # nothing runs and it still does not fit: it never will
-                    waiting.popleft()
-                    job.result = {"error": "larger than the simulation's device room"}
-                    job.t_done = time.perf_counter() - t0
-                    job.net.close()
-                    job.net = None
-                    done.append(job)
-                    continue
-                waiting.popleft()
-                job.t_admit = time.perf_counter() - t0
-                job.net.close()  # the simulation keeps what it needs
-                job.net = None
-                running[job.nid] = job
-            tb = time.perf_counter()
-            acc["admit_s"] += tb - ta
-            if running:
-                self.sim.step(self.chunk)
-                tc = time.perf_counter()
-                self.sim.synchronize()
-                td = time.perf_counter()
-                acc["launch_s"] += tc - tb
-                acc["gpu_wait_s"] += td - tc
-                acc["chunks"] += 1
-                acc["running_sum"] += len(running)
-                for nid in list(running):
-                    if self.sim.progress(nid)[1] == 0:
-                        job = running.pop(nid)
-                        state = read_result(self.sim, nid, job.n, job.n_syn)
-                        self.sim.remove(nid)
+            served_any = False
+            for ln in self.lanes:
+                while (nxt < len(arrivals) and arrivals[nxt].arrival <= time.perf_counter() - t0
+                       and len(building) + len(waiting) < self.max_ready):
+                    building.add(self.pool.submit(_build, arrivals[nxt]))
+                    nxt += 1
+                for fut in [f for f in finishing if f.done()]:
+                    finishing.remove(fut)
+                    done.append(fut.result())
+                for fut in [f for f in building if f.done()]:
+                    building.discard(fut)
+                    job = fut.result()
+                    job.t_built = time.perf_counter() - t0
+                    waiting.append(job)
+                if ln is in_flight:
+                    if len(self.lanes) > 1:
+                        continue  # the other lanes are serviced while this one runs
+                    tw = time.perf_counter()
+                    ln.sim.synchronize()
+                    in_flight = None
+                    acc["gpu_wait_s"] += time.perf_counter() - tw
+                # retirement: this lane has no chunk in flight
+                tr = time.perf_counter()
+                for nid in list(ln.running):
+                    if ln.sim.progress(nid)[1] == 0:
+                        job = ln.running.pop(nid)
+                        state = read_result(ln.sim, nid, job.n, job.n_syn)
+                        ln.sim.remove(nid)
                         finishing.append(self.finisher.submit(_finish, job, state, t0))
-                acc["retire_s"] += time.perf_counter() - td
-            else:
+                ta = time.perf_counter()
+                acc["retire_s"] += ta - tr
+                # admission in arrival order while this lane has room
+                while waiting:
+                    job = waiting[0]
+                    try:
+                        job.nid = ln.sim.add(job.net, job.params, job.drive, steps=job.steps)
+                    except NCError as e:
+                        if e.code != 5:  # anything but NC_ERR_CAPACITY
+                            raise
+                        if ln.running:  # wait for room
+                            break
+                        # an empty lane has no room for it: no lane ever will
+                        waiting.popleft()
+                        job.result = {"error": "larger than the simulation's device room"}
+                        job.t_done = time.perf_counter() - t0
+                        job.net.close()
+                        job.net = None
+                        done.append(job)
+                        continue
+                    waiting.popleft()
+                    job.t_admit = time.perf_counter() - t0
+                    job.net.close()  # the simulation keeps what it needs
+                    job.net = None
+                    ln.running[job.nid] = job
+                tb = time.perf_counter()
+                acc["admit_s"] += tb - ta
+                # the GPU finishes the other lane's chunk, then takes this one
+                if in_flight is not None:
+                    in_flight.sim.synchronize()
+                    in_flight = None
+                tc = time.perf_counter()
+                acc["gpu_wait_s"] += tc - tb
+                if ln.running:
+                    ln.sim.step(self.chunk)
+                    in_flight = ln
+                    served_any = True
+                    acc["chunks"] += 1
+                    acc["running_sum"] += len(ln.running)
+                acc["launch_s"] += time.perf_counter() - tc
+            if not served_any and in_flight is None:
+                tb = time.perf_counter()
                 wait = 0.001
                 if nxt < len(arrivals) and not building:
                     wait = max(0.0, min(0.01, arrivals[nxt].arrival - (time.perf_counter() - t0)))
@@ -201,23 +238,31 @@ class Server:
             if log and time.perf_counter() - last_log > 10.0:
                 last_log = time.perf_counter()
                 log("served %d / %d jobs, %d running, %d waiting, %d arrived, %.1fs" %
-                    (len(done), len(arrivals), len(running), len(waiting),
+                    (len(done), len(arrivals), n_running(), len(waiting),
                      sum(1 for j in arrivals if j.arrival <= last_log - t0), last_log - t0))
-        summary = {"unfinished": len(arrivals) - len(done), "running": len(running), "waiting": len(waiting),
+        if in_flight is not None:
+            in_flight.sim.synchronize()
+        summary = {"unfinished": len(arrivals) - len(done), "running": n_running(), "waiting": len(waiting),
+                   "lanes": len(self.lanes),
                    "loop": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in acc.items()}}
         if acc["chunks"]:
-            summary["loop"]["mean_running"] = acc["running_sum"] / acc["chunks"]
+            summary["loop"]["mean_running_per_chunk"] = acc["running_sum"] / acc["chunks"]
         if kernel_every:
             for kind, name in ((1, "k_neuron_us"), (3, "k_fanout_us")):
-                n, ms = self.sim.kernel_timing(kind)
+                n = ms = 0
+                for ln in self.lanes:
+                    a, b = ln.sim.kernel_timing(kind)
+                    n, ms = n + a, ms + b
                 summary["loop"][name] = 1000.0 * ms / n if n else None
-            self.sim.set_kernel_timing(0)
+            for ln in self.lanes:
+                ln.sim.set_kernel_timing(0)
         return done, summary
 
     def close(self):
         self.pool.shutdown(wait=True)
         self.finisher.shutdown(wait=True)
-        self.sim.close()
+        for ln in self.lanes:
+            ln.sim.close()
 
 
 def run_alone(spec):

@@ -16,14 +16,15 @@ def say(*a):
     sys.stdout.flush()
 
 
-def test_served_jobs_equal_alone_under_queueing():
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_served_jobs_equal_alone_under_queueing(lanes):
     specs = SV.served_specs(10, seed=11, n_range=(300, 2500), k_range=(20, 40), steps_range=(150, 700))
     rng = np.random.default_rng(5)
     arrivals = np.concatenate([[0.0] * 4, np.cumsum(rng.exponential(0.02, size=len(specs) - 4))])
     jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
     # room for two jobs at a time: four arrive together, so some wait for room
-    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=2, chunk=64, builders=4)
-    say("serving", len(jobs), "jobs")
+    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=2, chunk=64, builders=4, lanes=lanes)
+    say("serving", len(jobs), "jobs on", lanes, "lane(s)")
     done, rest = server.run(jobs, t_limit=120)
     server.close()
     assert rest["unfinished"] == 0 and len(done) == len(jobs)
