@@ -73,6 +73,13 @@ typedef struct nc_wiring_spec {
  * (src/NeuCor.cpp:17-42, :418-444) for the workload wiring rule. */
 nc_status nc_net_generate(const nc_wiring_spec* spec, nc_net** out);
 
+/* The same network, bit for bit, built on GPU `device`: positions, a cell
+ * grid and the rows of sources within the radius, sorted by source id, then
+ * the per-synapse delays, weights and sign classes (spec->n_threads is
+ * ignored).  Also replaces Neuron::makeConnections' all-pairs scan
+ * (src/NeuCor.cpp:418-444). */
+nc_status nc_net_generate_device(const nc_wiring_spec* spec, int32_t device, nc_net** out);
+
 /* Replaces NeuCor::createNeuron / NeuCor::createSynapse
  * (src/NeuCor.cpp:154-195): an explicit network.  xyz may be NULL.
  * Synapse k goes from[k] -> to[k]; duplicates of an existing (from,to) pair

@@ -75,6 +75,7 @@ SIGNATURES = {
     "nc_sim_create": (C.c_int, [C.POINTER(SimSpec), C.POINTER(P)]),
     "nc_net_device_size": (C.c_int, [P, u64p, u64p]),
     "nc_net_prepare": (C.c_int, [P]),
+    "nc_net_generate_device": (C.c_int, [C.POINTER(WiringSpec), C.c_int32, C.POINTER(P)]),
     "nc_sim_destroy": (None, [P]),
     "nc_sim_footprint": (C.c_int, [C.c_uint64, C.c_uint64, u64p]),
     "nc_sim_free_bytes": (C.c_int, [P, u64p]),

@@ -7,6 +7,7 @@
 // exact multiple of dt and every transcendental it evaluates becomes a table
 // indexed by a step difference (tables are built on the host from its libm).
 #pragma once
+#include <math.h>
 #include <stdint.h>
 
 #if defined(__HIP__)
@@ -93,3 +94,31 @@ NC_HD nc_u32x4 nc_draw(uint64_t seed, uint32_t a, uint32_t b, uint32_t stream) {
 
 // Uniform in [0,1) with 24 random bits: exact in float.
 NC_HD float nc_u01(uint32_t u) { return (float)(u >> 8) * (1.0f / 16777216.0f); }
+
+// ---- the workload wiring's cell grid (host and device builders alike) -----
+// 10 bits of x spread to every third bit (Morton order of 3-D cells).
+NC_HD uint32_t nc_part1by2(uint32_t x) {
+  x &= 0x3ff;
+  x = (x | (x << 16)) & 0x030000ff;
+  x = (x | (x << 8)) & 0x0300f00f;
+  x = (x | (x << 4)) & 0x030c30c3;
+  x = (x | (x << 2)) & 0x09249249;
+  return x;
+}
+
+// Cells per axis: the largest count whose cell side is at least the radius
+// (so the 27 cells around a neuron's cell hold every neighbour), at most 1024.
+NC_HD int nc_cells_per_axis(float side, float radius) {
+  int nc = (int)floor((double)side / (double)radius);
+  if (nc < 1) nc = 1;
+  while (nc > 1 && (double)side / nc < (double)radius) --nc;
+  if (nc > 1024) nc = 1024;
+  return nc;
+}
+
+NC_HD int nc_cell_of(float x, int ncell, float side) {
+  int c = (int)floor((double)x * (double)ncell / (double)side);
+  if (c < 0) c = 0;
+  if (c > ncell - 1) c = ncell - 1;
+  return c;
+}

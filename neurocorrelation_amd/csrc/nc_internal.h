@@ -36,6 +36,7 @@ namespace nc {
 void set_error(const std::string& msg);
 nc_status fail(nc_status code, const std::string& msg);
 void finish_net_stats(nc_net* net);
+nc_status check_wiring_spec(const nc_wiring_spec& s);
 
 // Input selection shared by the engine and nc_net_select_inputs.
 struct InputSel {

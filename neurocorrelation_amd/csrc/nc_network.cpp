@@ -35,31 +35,6 @@ void finish_net_stats(nc_net* net) {
     if (d > net->max_delay) net->max_delay = d;
 }
 
-static uint32_t part1by2(uint32_t x) {
-  x &= 0x3ff;
-  x = (x | (x << 16)) & 0x030000ff;
-  x = (x | (x << 8)) & 0x0300f00f;
-  x = (x | (x << 4)) & 0x030c30c3;
-  x = (x | (x << 2)) & 0x09249249;
-  return x;
-}
-
-// Number of cells per axis: the largest count whose cell side is >= radius.
-static int cells_per_axis(float side, float radius) {
-  int nc = (int)floor((double)side / (double)radius);
-  if (nc < 1) nc = 1;
-  while (nc > 1 && (double)side / nc < (double)radius) --nc;
-  if (nc > 1024) nc = 1024;
-  return nc;
-}
-
-static int cell_of(float x, int ncell, float side) {
-  int c = (int)floor((double)x * (double)ncell / (double)side);
-  if (c < 0) c = 0;
-  if (c > ncell - 1) c = ncell - 1;
-  return c;
-}
-
 nc_status select_inputs(const nc_net& net, const nc_drive_spec& d, InputSel* out) {
   if (d.n_inputs > net.n) return fail(NC_ERR_ARG, "n_inputs exceeds the neuron count");
   if (d.n_correlated > d.n_inputs) return fail(NC_ERR_ARG, "n_correlated exceeds n_inputs");
@@ -90,15 +65,25 @@ nc_status select_inputs(const nc_net& net, const nc_drive_spec& d, InputSel* out
 
 using nc::fail;
 
-extern "C" nc_status nc_net_generate(const nc_wiring_spec* spec, nc_net** out) {
-  if (!spec || !out) return fail(NC_ERR_ARG, "null argument");
-  *out = nullptr;
-  const nc_wiring_spec s = *spec;
+namespace nc {
+nc_status check_wiring_spec(const nc_wiring_spec& s) {
   if (s.n_neurons == 0) return fail(NC_ERR_ARG, "n_neurons must be > 0");
   if (!(s.side > 0.0f) || !(s.radius > 0.0f)) return fail(NC_ERR_ARG, "side and radius must be > 0");
   if (s.delay_min < 1 || s.delay_max < s.delay_min || s.delay_max > 250)
     return fail(NC_ERR_ARG, "delays must satisfy 1 <= delay_min <= delay_max <= 250");
   if (!(s.w_max >= s.w_min)) return fail(NC_ERR_ARG, "w_max < w_min");
+  return NC_OK;
+}
+}  // namespace nc
+
+extern "C" nc_status nc_net_generate(const nc_wiring_spec* spec, nc_net** out) {
+  if (!spec || !out) return fail(NC_ERR_ARG, "null argument");
+  *out = nullptr;
+  const nc_wiring_spec s = *spec;
+  {
+    nc_status st = nc::check_wiring_spec(s);
+    if (st != NC_OK) return st;
+  }
   const uint32_t n = s.n_neurons;
 
   nc_net* net = new (std::nothrow) nc_net();
@@ -112,12 +97,12 @@ extern "C" nc_status nc_net_generate(const nc_wiring_spec* spec, nc_net** out) {
       py[i] = nc_u01(r.v[1]) * s.side;
       pz[i] = nc_u01(r.v[2]) * s.side;
     }
-    const int ncell = nc::cells_per_axis(s.side, s.radius);
+    const int ncell = nc_cells_per_axis(s.side, s.radius);
     std::vector<uint32_t> cx(n), cy(n), cz(n);
     for (uint32_t i = 0; i < n; ++i) {
-      cx[i] = nc::cell_of(px[i], ncell, s.side);
-      cy[i] = nc::cell_of(py[i], ncell, s.side);
-      cz[i] = nc::cell_of(pz[i], ncell, s.side);
+      cx[i] = nc_cell_of(px[i], ncell, s.side);
+      cy[i] = nc_cell_of(py[i], ncell, s.side);
+      cz[i] = nc_cell_of(pz[i], ncell, s.side);
     }
     // Neuron ids: draw order, or Morton order of the cells (ties by draw index).
     std::vector<uint32_t> order(n);
@@ -125,7 +110,7 @@ extern "C" nc_status nc_net_generate(const nc_wiring_spec* spec, nc_net** out) {
     if (s.spatial_order) {
       std::vector<uint64_t> key(n);
       for (uint32_t i = 0; i < n; ++i) {
-        uint64_t m = nc::part1by2(cx[i]) | (nc::part1by2(cy[i]) << 1) | (nc::part1by2(cz[i]) << 2);
+        uint64_t m = nc_part1by2(cx[i]) | (nc_part1by2(cy[i]) << 1) | (nc_part1by2(cz[i]) << 2);
         key[i] = (m << 32) | i;
       }
       std::sort(key.begin(), key.end());

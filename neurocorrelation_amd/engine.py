@@ -31,11 +31,16 @@ class Network:
 
     @classmethod
     def generate(cls, n_neurons, side, radius, delay_min=1, delay_max=20, w_min=0.2, w_max=0.8,
-                 inhib_frac=0.0, seed=0, spatial_order=1, n_threads=0):
+                 inhib_frac=0.0, seed=0, spatial_order=1, n_threads=0, device=None):
+        """The workload wiring rule on host threads, or (device = a GPU
+        index) on that GPU; both build the same network bit for bit."""
         spec = WiringSpec(n_neurons, spatial_order, seed, side, radius, delay_min, delay_max,
                           w_min, w_max, inhib_frac, n_threads)
         h = C.c_void_p()
-        check(lib().nc_net_generate(C.byref(spec), C.byref(h)))
+        if device is None:
+            check(lib().nc_net_generate(C.byref(spec), C.byref(h)))
+        else:
+            check(lib().nc_net_generate_device(C.byref(spec), int(device), C.byref(h)))
         return cls(h)
 
     @classmethod

@@ -48,3 +48,4 @@ def test_errors_cross_the_boundary_as_codes():
     assert b"null" in L.nc_last_error()
     assert L.nc_sim_step(None, 1) == 1
     assert L.nc_net_prepare(None) == 1
+    assert L.nc_net_generate_device(None, 0, None) == 1
