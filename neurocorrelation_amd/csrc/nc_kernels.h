@@ -9,8 +9,9 @@
 //             wave applies post-side STDP to the incoming synapses of each
 //             of its fired lanes, and turns the slice's spikes arriving at s
 //             (the pending list of s's window) into active-slot bits
-//   k_fanout  one wave per slice with fired lanes: Synapse::fire on their
-//             outgoing synapses (source-order latch copy, out->in slot map),
+//   k_fanout  a capped grid of waves, each taking every nwaves-th slice's
+//             fired record: Synapse::fire on the fired lanes' outgoing
+//             synapses (source-order latch copy, out->in slot map),
 //             appending accepted spikes to the target slice's pending list
 //             of their arrival's window
 // The kernel boundary orders the phases.  Every synapse slot has one writer
@@ -549,6 +550,12 @@ __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nwaves = gridDim.x * WPB;
   uint32_t n_acc = 0, n_out = 0;
+  // With one slice per wave and record i being slice act_q0 + i, the out-row
+  // heads of the wave's slice are loaded beside its fired record instead of
+  // after it.
+  const bool one = a.act_identity && nwaves >= a.n_active_slices;
+  uint2 row_pre = make_uint2(0u, 0u);
+  if (one && wave < a.n_active_slices) row_pre = a.out_row[(a.act_q0 + wave) * 64 + lane];
   // Interleaved: this wave's slices are wave, wave + nwaves, ... (a busy
   // stretch of slices spreads over all waves), 64 records per load.
   for (uint32_t fb = wave; fb < a.n_active_slices; fb += 64 * nwaves) {
@@ -565,7 +572,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
       // The out-lists of all fired lanes, flattened: lane-parallel row loads,
       // a wave prefix sum of out-degrees, then 128 edges per pass.
       uint2 row = make_uint2(0u, 0u);
-      if ((m >> lane) & 1ull) row = a.out_row[fq * 64 + lane];
+      if ((m >> lane) & 1ull) row = one ? row_pre : a.out_row[fq * 64 + lane];
       uint32_t incl = row.y;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
