@@ -288,3 +288,31 @@ def test_edge_networks(oracle_mod):
     with pytest.raises(nc.NCError):
         bad = nc.Network.from_edges(2, [1], [0], [0.8], [252])
         nc.Simulation.for_network(bad).add(bad, nc.default_params())
+
+
+def test_largest_in_degree(oracle_mod):
+    """A neuron with the largest supported in-degree (262140): a spike from
+    its last source (the slot furthest into its slice) arrives and charges it
+    exactly as in the oracle; one synapse more is refused."""
+    k = 262_140
+    src = np.arange(1, k + 1, dtype=np.uint32)
+    net = nc.Network.from_edges(k + 1, np.zeros(k, np.uint32), src, np.full(k, 0.8, np.float32),
+                                np.ones(k, np.uint8))
+    sim = nc.Simulation.for_network(net, raster_capacity=1024)
+    nid = sim.add(net, nc.default_params())
+    o = oracle_mod.Sim(net.export())
+    for x in (sim, o):
+        if x is sim:
+            x.push_forced(nid, [1, 1], [k, k - 1], [0, 0])
+        else:
+            x.push_forced([1, 1], [k, k - 1], [0, 0])
+    sim.step(6)
+    o.step(6)
+    assert_bits(sim.potential(nid, k + 1), o.state()["V"], "V")
+    assert_bits(sim.weights(nid, k), o.state()["w"], "w")
+    too_many = nc.Network.from_edges(k + 2, np.zeros(k + 1, np.uint32), np.arange(1, k + 2, dtype=np.uint32),
+                                     np.full(k + 1, 0.8, np.float32), np.ones(k + 1, np.uint8))
+    sim2 = nc.Simulation.for_network(too_many)
+    with pytest.raises(nc.NCError) as e:
+        sim2.add(too_many, nc.default_params())
+    assert e.value.code == 1
