@@ -286,6 +286,7 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   const uint32_t pos = q * 64 + lane;
   uint2 nw = a.ndw[pos];
   float V = a.V[pos];
+  const uint32_t V_in = __float_as_uint(V);
   int32_t lf = a.last_fire[pos];
   const uint2 pc = a.pend_cnt[q];
   uint32_t sm = a.asum[pos];
@@ -427,7 +428,8 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   }
 
   if (valid) {
-    a.V[pos] = V;
+    // a neuron at rest stays at the base level bit for bit: no store
+    if (__float_as_uint(V) != V_in) a.V[pos] = V;
     if (fire) a.last_fire[pos] = lf;
   }
   uint32_t n_post = 0;
