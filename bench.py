@@ -273,7 +273,10 @@ def bench_served(args):
     lat = np.array([j.t_done - j.arrival for j in done])
     t_end = max(j.t_done for j in done) if done else 0.0
     syn_steps = sum(j.n_syn * j.steps for j in done)
-    # the same jobs alone, one after another, on the plain path (a sample of the completed ones)
+    # the same jobs alone, one after another, on the plain path: a sample of
+    # the completed ones drawn by job index (the same jobs whatever order
+    # they completed in)
+    done = sorted(done, key=lambda j: j.idx)
     sample = sorted(rng.choice(len(done), size=min(args.sample, len(done)), replace=False).tolist()) if done else []
     seq_t = []
     for i in sample:
