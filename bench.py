@@ -183,10 +183,14 @@ def bench_batched(args):
 
     sim.synchronize()
     t_start = time.perf_counter()
-    chosen = sim.tune(args.tune_steps)
-    done = 2 * 16 * args.tune_steps
-    print("[bench] tuned launch %s after %d steps (%.2fs)" % (chosen, done, time.perf_counter() - t_start),
-          file=sys.stderr, flush=True)
+    if args.tune_steps:
+        chosen = sim.tune(args.tune_steps)
+        done = 2 * 32 * args.tune_steps
+        print("[bench] tuned launch %s after %d steps (%.2fs)" % (chosen, done, time.perf_counter() - t_start),
+              file=sys.stderr, flush=True)
+    else:
+        sim.set_launch(args.waves_per_block)
+        chosen, done = sim.launch(), 0
     while done < steps:
         k = min(args.chunk, steps - done)
         sim.step(k)
@@ -223,7 +227,7 @@ def bench_batched(args):
         assert np.array_equal(alone.firings(a, n_neu[j]), f), "firings differ, net %d" % j
         alone.close()
         net.close()
-    seq_per_sim = float(np.mean(seq_t))
+    seq_per_sim = float(np.mean(seq_t)) if seq_t else None
     sims_s = len(built) / elapsed
     out = {
         "metric": "batched: sims/s and synapse-updates/s vs sequential plain runs",
@@ -234,9 +238,9 @@ def bench_batched(args):
                    "synapses": S_tot, "steps_per_network": steps, "launch": chosen,
                    "tune_steps_per_trial": args.tune_steps},
         "synapse_updates_per_s": S_tot * steps / elapsed,
-        "sequential_plain": {"sample": len(sample), "mean_s_per_sim": seq_per_sim, "sims_per_s": 1.0 / seq_per_sim,
-                             "identical_final_state": True},
-        "speedup_vs_sequential": sims_s * seq_per_sim,
+        "sequential_plain": {"sample": len(sample), "mean_s_per_sim": seq_per_sim,
+                             "sims_per_s": 1.0 / seq_per_sim if seq_per_sim else None, "identical_final_state": True},
+        "speedup_vs_sequential": sims_s * seq_per_sim if seq_per_sim else None,
         "setup_s": setup_s,
     }
     print(json.dumps(out), flush=True)
@@ -326,7 +330,9 @@ def main():
                     help="skip the counting replay (profiler runs: every launch is then uninstrumented)")
     ap.add_argument("--nets", type=int, default=4096, help="batched-* workloads: number of networks")
     ap.add_argument("--sample", type=int, default=12, help="batched-*: networks rerun on the plain path")
-    ap.add_argument("--tune-steps", type=int, default=16, help="batched-*: steps per autotuner trial")
+    ap.add_argument("--tune-steps", type=int, default=16,
+                    help="batched-*: steps per autotuner trial (0: no autotuning, --waves-per-block launch)")
+    ap.add_argument("--waves-per-block", type=int, default=4, help="batched-*: launch shape when not autotuning")
     ap.add_argument("--rate", type=float, default=200.0, help="served: job arrivals per second")
     ap.add_argument("--window", type=float, default=60.0, help="served: seconds of arrivals")
     ap.add_argument("--max-neurons", type=int, default=100_000, help="served: largest job network")
