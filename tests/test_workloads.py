@@ -1,0 +1,64 @@
+"""CPU: the workload generators behind bench.py (sizes, degrees, step budgets
+and drives of the batched and served specs; deterministic per seed), and the
+benchmark's algorithmic byte model of the neuron-step kernel."""
+import importlib.util
+import math
+import os
+
+import numpy as np
+
+from neurocorrelation_amd import serve as SV
+from neurocorrelation_amd import workloads as W
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_batched_specs_ranges_and_determinism():
+    a = W.batched_specs(count=200, seed=3)
+    b = W.batched_specs(count=200, seed=3)
+    assert a == b
+    ns = np.array([s["wiring"]["n_neurons"] for s in a])
+    ks = np.array([s["k"] for s in a])
+    assert ns.min() >= 200 and ns.max() <= 20_000
+    assert ks.min() >= 10 and ks.max() <= 60
+    # log-uniform sizes: the median sits near the geometric mean of the range
+    assert abs(math.log(np.median(ns)) - 0.5 * (math.log(200) + math.log(20_000))) < 0.6
+    for s in a:
+        d, w = s["drive"], s["wiring"]
+        assert d["n_inputs"] >= 6 and d["n_correlated"] % 2 == 0 and d["n_correlated"] <= d["n_inputs"]
+        assert 0.0 <= d["amp_corr"] <= 1.0 and d["seed"] == w["seed"]
+        assert w["side"] == W.side_for(w["n_neurons"]) and s["steps"] == 20_000
+
+
+def test_served_specs_ranges_and_weights():
+    a = SV.served_specs(300, seed=3)
+    assert a == SV.served_specs(300, seed=3)
+    ns = np.array([s["wiring"]["n_neurons"] for s in a])
+    st = np.array([s["steps"] for s in a])
+    assert ns.min() >= 300 and ns.max() <= 100_000
+    assert st.min() >= 1_000 and st.max() <= 50_000
+    assert all(s["wiring"]["inhib_frac"] == 0.2 and s["wiring"]["w_max"] == 0.3 for s in a)
+    run = SV.served_specs(5, seed=1, weights=(0.2, 0.8, 0.0))
+    assert all(s["wiring"]["inhib_frac"] == 0.0 and s["wiring"]["w_min"] == 0.2 for s in run)
+
+
+def test_radius_gives_the_bulk_degree():
+    # density x sphere volume at the radius is the wanted out-degree
+    for k in (10, 50, 100):
+        r = W.radius_for_degree(k)
+        assert abs(W.DENSITY * 4.0 / 3.0 * math.pi * r ** 3 - k) < 1e-9
+
+
+def test_neuron_kernel_byte_model():
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    keys = ("active_slot_steps", "arrivals", "fired", "fired_in_degree", "pending_scanned", "mask_words_read",
+            "mask_words_written")
+    c0 = {k: 0 for k in keys}
+    c1 = {"active_slot_steps": 100, "arrivals": 10, "fired": 5, "fired_in_degree": 50, "pending_scanned": 200,
+          "mask_words_read": 30, "mask_words_written": 20}
+    b, ev = bench.neuron_kernel_bytes(640, c0, c1, 10)
+    expect = (20 * 640 + 40 * 10 + 4 * (3 + 2) + 16 * 10 + 16 * 1 + 20 * 5 + 12 * 0.5 + 4 * 20)
+    assert abs(b - expect) < 1e-9
+    assert ev["A"] == 10 and ev["F_in"] == 5
