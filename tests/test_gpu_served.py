@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 from neurocorrelation_amd import serve as SV
+from neurocorrelation_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
@@ -46,3 +47,35 @@ def test_job_larger_than_the_room_is_refused():
     assert rest["unfinished"] == 0
     assert "error" in by[1].result
     assert "hash" in by[0].result and "hash" in by[2].result
+
+
+def test_served_jobs_match_the_oracle(oracle_mod):
+    """Two jobs of the served stream (seeded drive, 20% inhibitory weights,
+    delays up to 32 steps), run in the served mix, equal the oracle's run of
+    each job: firings, final potentials and weights."""
+    import hashlib
+
+    specs = SV.served_specs(2, seed=21, n_range=(700, 1800), k_range=(40, 90), steps_range=(1500, 2500))
+    jobs = [SV.Job(i, sp, 0.0) for i, sp in enumerate(specs)]
+    server = SV.Server(max_neurons=8000, max_synapses=400_000, max_nets=4, chunk=100, builders=2, lanes=2)
+    done, rest = server.run(jobs, t_limit=120)
+    server.close()
+    assert rest["unfinished"] == 0
+    for job in sorted(done, key=lambda j: j.idx):
+        net, drive, params, steps = W.build_spec(job.spec)
+        ex = net.export()
+        o = oracle_mod.Sim(ex, dt=params.dt, lr=params.learning_rate, pre_f=params.pre_factor,
+                           post_f=params.post_factor)
+        ids, groups = net.select_inputs(drive)
+        ncorr = drive.n_groups if drive.n_correlated > 0 else 0
+        o.set_drive(ids, groups, ncorr, drive.period, drive.p0, drive.amp_corr, drive.amp_ind, drive.bg_prob,
+                    drive.seed)
+        say("oracle: job", job.idx, "n", net.n, "steps", steps)
+        o.step(steps, cap=net.n * steps)
+        st = o.state()
+        h = hashlib.sha256()
+        h.update(st["V"].astype(np.float32).tobytes())
+        h.update(st["w"].astype(np.float32).tobytes())
+        h.update(st["firings"].astype(np.uint32).tobytes())
+        assert job.result == {"spikes": int(st["firings"].sum()), "hash": h.hexdigest()}, job.idx
+        net.close()
