@@ -402,6 +402,7 @@ struct nc_sim {
   nc_slice_rec* active = nullptr;
   uint32_t *forced_in = nullptr, *forced_bg = nullptr;
   uint4* fired = nullptr;
+  uint32_t* fired_cnt = nullptr;
 
   uint64_t ctr_rows = 0;
   uint32_t *raster = nullptr, *raster_cnt = nullptr;
@@ -516,6 +517,7 @@ nc_step_args base_args(nc_sim* sim) {
   a.pend1 = sim->pend1;
   a.pend_cnt = sim->pend_cnt;
   a.fired = sim->fired;
+  a.fired_cnt = sim->fired_cnt;
   return a;
 }
 
@@ -698,7 +700,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
     H(hipMalloc(&sim->forced_in, (P / 32) * 4), "hipMalloc forced_in");
     H(hipMalloc(&sim->forced_bg, (P / 32) * 4), "hipMalloc forced_bg");
-    H(hipMalloc(&sim->fired, Q * sizeof(uint4)), "hipMalloc fired");
+    H(hipMalloc(&sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)), "hipMalloc fired");
+    H(hipMalloc(&sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4), "hipMalloc fired_cnt");
 
     H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
     if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
@@ -714,7 +717,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
     H(hipMemsetAsync(sim->ndw, 0, P * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
-    H(hipMemsetAsync(sim->fired, 0, Q * sizeof(uint4), sim->stream), "memset");
+    H(hipMemsetAsync(sim->fired, 0, (Q + NC_FLISTS * 8ull) * sizeof(uint4), sim->stream), "memset");
+    H(hipMemsetAsync(sim->fired_cnt, 0, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
@@ -746,7 +750,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
                   sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
-                  sim->forced_bg, sim->fired, sim->raster, sim->raster_cnt,
+                  sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p, sim->d_wbuf.p, sim->d_stage.p};
   for (void* p : ptrs)
@@ -1135,7 +1139,10 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
   // k_fanout scans the fired records with at most fanout_waves waves (default
   // 16384, 64 per CU): a step's fired slices are few.
   const uint32_t fan_waves = sim->cfg.fanout_waves ? sim->cfg.fanout_waves : 16384u;
-  const uint32_t fan_grid = std::min<uint32_t>(grid, (fan_waves + wpb - 1) / wpb);
+  // at least one wave per fired list
+  const uint32_t fan_grid = std::max<uint32_t>(std::min<uint32_t>(grid, (fan_waves + wpb - 1) / wpb),
+                                               (NC_FLISTS + wpb - 1) / wpb);
+  a.fcap = ((grid + NC_FLISTS - 1) / NC_FLISTS) * wpb;
   const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
   const void* fn_neuron = nullptr;
   const void* fn_fanout = nullptr;
@@ -1146,6 +1153,9 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
     default: fn_neuron = (const void*)k_neuron<4>, fn_fanout = (const void*)k_fanout<4>; break;
   }
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, sim->d_chunk_off, 1u, 0u);
+  NC_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_zero_strided, dim3(1), dim3(2 * NC_FLISTS), 0, st, sim->fired_cnt, 2u * NC_FLISTS,
+                     (uint32_t)NC_FCNT_STRIDE);
   NC_HIP(hipGetLastError());
   size_t nev_used = 0;
   std::vector<int> timed_kinds;

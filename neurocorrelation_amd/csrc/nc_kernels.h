@@ -36,6 +36,10 @@
 // (slot - first slot of the slice) << 8 | (arrival & 255): a list scanned at
 // step s holds no arrival before s nor past s + 255 (delays are 1..255).
 #define NC_PEND_LOG_W 4
+// Slices with fired lanes go to one of NC_FLISTS lists (by workgroup), so
+// that the appends of a step spread over that many counters.
+#define NC_FLISTS 64
+#define NC_FCNT_STRIDE 32  // words between two counters (one cache line each)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay below 2^24 (64 lanes x 65535 chunks of 4)
 #define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
 
@@ -98,7 +102,9 @@ struct nc_step_args {
   uint32_t* forced_in;
   uint32_t* forced_bg;
   int32_t use_forced;
-  uint4* fired;  // per active slice (by active-list index): fired-lane mask lo/hi, step, slice
+  uint4* fired;  // slices with fired lanes in NC_FLISTS lists (list b at b * fcap): mask lo/hi, step, slice
+  uint32_t* fired_cnt;  // list lengths: [parity of the step in the call][list], NC_FCNT_STRIDE apart
+  uint32_t fcap;        // entries per list
   uint32_t* raster;           // (pos, step) pairs
   uint32_t* raster_cnt;
   uint32_t raster_cap;
@@ -189,6 +195,9 @@ __global__ void k_advance(uint32_t* chunk_off, uint32_t steps) {
 }
 __global__ void k_set_u32(uint32_t* p, uint32_t n, uint32_t v) {
   if (threadIdx.x < n) p[threadIdx.x] = v;
+}
+__global__ void k_zero_strided(uint32_t* p, uint32_t n, uint32_t stride) {
+  if (threadIdx.x < n) p[threadIdx.x * stride] = 0u;
 }
 
 // The weights of a run of slot records, packed 4 bytes per slot (host reads
@@ -434,8 +443,18 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   }
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
-  if (wave_in && lane == 0)
-    a.fired[wave] = fm ? make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q) : make_uint4(0u, 0u, 0u, 0u);
+  {
+    // This step's lists get the fired slices; the other parity's lists,
+    // which served the previous step's fan-out, are cleared for the next.
+    const uint32_t par = (uint32_t)ci & 1u;
+    if (blk == 0 && threadIdx.x < NC_FLISTS)
+      a.fired_cnt[((par ^ 1u) * NC_FLISTS + threadIdx.x) * NC_FCNT_STRIDE] = 0u;
+    if (fm && lane == 0) {
+      const uint32_t b = blockIdx.x % NC_FLISTS;  // workgroups running together use different lists
+      const uint32_t at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], 1u);
+      a.fired[b * a.fcap + at] = make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q);
+    }
+  }
   if (fm) {
     // The row passes below read the fired lanes' slot records from other
     // lanes; when a fired lane wrote some of them above (arrival STDP), make
@@ -552,17 +571,18 @@ __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
   const uint32_t lane = threadIdx.x & 63;
   const uint32_t nwaves = gridDim.x * WPB;
   uint32_t n_acc = 0, n_out = 0;
-  // With one slice per wave and record i being slice act_q0 + i, the out-row
-  // heads of the wave's slice are loaded beside its fired record instead of
-  // after it.
-  const bool one = a.act_identity && nwaves >= a.n_active_slices;
-  uint2 row_pre = make_uint2(0u, 0u);
-  if (one && wave < a.n_active_slices) row_pre = a.out_row[(a.act_q0 + wave) * 64 + lane];
-  // Interleaved: this wave's slices are wave, wave + nwaves, ... (a busy
-  // stretch of slices spreads over all waves), 64 records per load.
-  for (uint32_t fb = wave; fb < a.n_active_slices; fb += 64 * nwaves) {
-    const uint32_t fi = fb + lane * nwaves;
-    const uint4 fl = fi < a.n_active_slices ? a.fired[fi] : make_uint4(0u, 0u, 0u, 0u);
+  // Wave w takes list w % NC_FLISTS, entries w / NC_FLISTS, + per, ... (64
+  // records per load).
+  const uint32_t per = nwaves / NC_FLISTS;
+  const uint32_t lst = wave % NC_FLISTS, first = wave / NC_FLISTS;
+  const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;
+  const uint32_t nf = first < per ? __builtin_amdgcn_readfirstlane(
+                                        a.fired_cnt[((((uint32_t)ci & 1u) * NC_FLISTS) + lst) * NC_FCNT_STRIDE])
+                                  : 0u;
+  const uint4* const flist = a.fired + lst * a.fcap;
+  for (uint32_t fb = first; fb < nf; fb += 64 * per) {
+    const uint32_t fi = fb + lane * per;
+    const uint4 fl = fi < nf ? flist[fi] : make_uint4(0u, 0u, 0u, 0u);
     unsigned long long has = __ballot((fl.x | fl.y) != 0u);
     while (has) {
       const uint32_t fsrc = (uint32_t)__builtin_ctzll(has);
@@ -574,7 +594,7 @@ __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
       // The out-lists of all fired lanes, flattened: lane-parallel row loads,
       // a wave prefix sum of out-degrees, then 128 edges per pass.
       uint2 row = make_uint2(0u, 0u);
-      if ((m >> lane) & 1ull) row = one ? row_pre : a.out_row[fq * 64 + lane];
+      if ((m >> lane) & 1ull) row = a.out_row[fq * 64 + lane];
       uint32_t incl = row.y;
 #pragma unroll
       for (int o = 1; o < 64; o <<= 1) {
