@@ -1136,13 +1136,14 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
   }
   const uint32_t wpb = sim->cfg.waves_per_block;
   const uint32_t grid = (a.n_active_slices + wpb - 1) / wpb;
-  // k_fanout scans the fired records with at most fanout_waves waves (default
-  // 16384, 64 per CU): a step's fired slices are few.
-  const uint32_t fan_waves = sim->cfg.fanout_waves ? sim->cfg.fanout_waves : 16384u;
-  // at least one wave per fired list
-  const uint32_t fan_grid = std::max<uint32_t>(std::min<uint32_t>(grid, (fan_waves + wpb - 1) / wpb),
-                                               (NC_FLISTS + wpb - 1) / wpb);
-  a.fcap = ((grid + NC_FLISTS - 1) / NC_FLISTS) * wpb;
+  // A fired list holds at most fcap records (one per k_neuron wave of its
+  // residue).  k_fanout gives each list as many readers as that, so that a
+  // record is one wave's work, up to fanout_waves waves in all (default
+  // 16384, 64 per CU): then each wave takes every (waves / 64)-th record.
+  a.fcap = (grid * wpb + NC_FLISTS - 1) / NC_FLISTS;
+  const uint32_t fan_waves = std::min<uint32_t>(sim->cfg.fanout_waves ? sim->cfg.fanout_waves : 16384u,
+                                                a.fcap * NC_FLISTS);
+  const uint32_t fan_grid = std::max<uint32_t>((fan_waves + wpb - 1) / wpb, (NC_FLISTS + wpb - 1) / wpb);
   const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
   const void* fn_neuron = nullptr;
   const void* fn_fanout = nullptr;

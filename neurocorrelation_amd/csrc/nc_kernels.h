@@ -36,7 +36,7 @@
 // (slot - first slot of the slice) << 8 | (arrival & 255): a list scanned at
 // step s holds no arrival before s nor past s + 255 (delays are 1..255).
 #define NC_PEND_LOG_W 4
-// Slices with fired lanes go to one of NC_FLISTS lists (by workgroup), so
+// Slices with fired lanes go to one of NC_FLISTS lists (by wave index), so
 // that the appends of a step spread over that many counters.
 #define NC_FLISTS 64
 #define NC_FCNT_STRIDE 32  // words between two counters (one cache line each)
@@ -450,7 +450,7 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
     if (blk == 0 && threadIdx.x < NC_FLISTS)
       a.fired_cnt[((par ^ 1u) * NC_FLISTS + threadIdx.x) * NC_FCNT_STRIDE] = 0u;
     if (fm && lane == 0) {
-      const uint32_t b = blockIdx.x % NC_FLISTS;  // workgroups running together use different lists
+      const uint32_t b = wave % NC_FLISTS;  // waves running together use different lists
       const uint32_t at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], 1u);
       a.fired[b * a.fcap + at] = make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q);
     }
