@@ -5,9 +5,9 @@
 //   neurons are renumbered into "positions": windows of 1024 ids are sorted
 //   by in-degree (SELL-C-sigma, C = 64, sigma = 1024) and cut into slices of
 //   64 lanes, one wave per slice.  The incoming synapses of a slice are
-//   stored chunk-major: chunk c of lane l holds the lane's slots 4c..4c+3 at
-//   slot index slice_off + c*256 + l*4, so a fired lane's row is read by the
-//   wave as 16 full 64-byte records per 64 slots.  Slot order inside a lane
+//   stored chunk-major: chunk c of lane l holds the lane's slots 8c..8c+7 at
+//   slot index slice_off + c*512 + l*8, so a fired lane's row is read by the
+//   wave as 8 full 128-byte lines per 64 slots.  Slot order inside a lane
 //   is ascending source id.
 //   slots  : 16 B per slot {w, depol, last_arr, 2*arrival + inhibitory}
 //   amask  : 1 bit per slot, set while the slot's spike is in its release
@@ -102,7 +102,7 @@ struct RangeAlloc {
 struct Layout {
   uint32_t n = 0, nq = 0;
   std::vector<uint32_t> pos_of_id, id_of_pos;  // id_of_pos: nq*64 (~0u = padding)
-  std::vector<uint32_t> slice_slots;           // slots per slice (multiple of 256)
+  std::vector<uint32_t> slice_slots;           // slots per slice (multiple of 64 * NC_CHUNK)
   std::vector<uint32_t> slice_mwords;          // active-mask words per slice (multiple of 64)
   uint64_t nslots = 0, nmwords = 0;
 };
@@ -135,8 +135,8 @@ void make_layout(const nc_net& net, Layout* L) {
       uint32_t d = (uint32_t)(net.in_ptr[id + 1] - net.in_ptr[id]);
       mx = std::max(mx, d);
     }
-    uint32_t nch = (mx + 3) / 4;
-    L->slice_slots[q] = nch * 256;
+    uint32_t nch = (mx + NC_CHUNK - 1) / NC_CHUNK;
+    L->slice_slots[q] = nch * 64 * NC_CHUNK;
     L->nslots += L->slice_slots[q];
     L->slice_mwords.push_back(((mx + 31) / 32) * 64);
     L->nmwords += L->slice_mwords.back();
@@ -236,7 +236,7 @@ std::shared_ptr<const NetImage> build_image(const nc_net& net) {
     const uint32_t q = pl / 64, l = pl % 64;
     const int64_t b = net.in_ptr[id], e = net.in_ptr[id + 1];
     for (int64_t k = 0; k < e - b; ++k) {
-      const uint32_t j = I.slice_off[q] + (uint32_t)((k >> 2) * 256 + l * 4 + (k & 3));
+      const uint32_t j = nc_slot_of(I.slice_off[q], l, (uint32_t)k);
       M.slot_of_syn[b + k] = j;
       I.slot_w[j] = net.w[b + k];
       if (net.inh[b + k]) I.slot_inh[j >> 5] |= 1u << (j & 31);
@@ -616,7 +616,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   if (!bytes) return fail(NC_ERR_ARG, "null argument");
   // the buffers nc_sim_create allocates for one network of this size
   const uint64_t P = (n_neurons + 63) / 64 * 64 + 64, Q = P / 64;
-  const uint64_t slots = (n_syn + n_syn / 4 + P * 4 + 255) / 256 * 256;  // padding allowance
+  const uint64_t slots = (n_syn + n_syn / 4 + P * NC_CHUNK + 255) / 256 * 256;  // padding allowance
   const uint64_t mwords = slots / 32 + 64 * Q;
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
            + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
@@ -674,7 +674,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
   sim->max_pos = (spec->max_neurons + 63) / 64 * 64 + 64 * (uint64_t)spec->max_nets;
   sim->max_slots = spec->max_slots
                        ? (spec->max_slots + 255) / 256 * 256
-                       : (spec->max_synapses + spec->max_synapses / 4 + sim->max_pos * 4 + 255) / 256 * 256;
+                       : (spec->max_synapses + spec->max_synapses / 4 + sim->max_pos * NC_CHUNK + 255) / 256 * 256;
   if (sim->max_slots >= (1ull << 32)) sim->max_slots = ((1ull << 32) - 256);
   sim->max_out = spec->max_synapses + 64;
   sim->max_nets = spec->max_nets;

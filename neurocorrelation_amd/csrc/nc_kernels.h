@@ -40,7 +40,11 @@
 // that the appends of a step spread over that many counters.
 #define NC_FLISTS 64
 #define NC_FCNT_STRIDE 32  // words between two counters (one cache line each)
-#define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay below 2^24 (64 lanes x 65535 chunks of 4)
+// In-slots are stored chunk-major: NC_CHUNK consecutive slots of a lane (one
+// 128-byte line of 16-byte records), then the next lane's.
+#define NC_CHUNK_LOG 3
+#define NC_CHUNK (1u << NC_CHUNK_LOG)
+#define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
 #define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
 
 // Per-network descriptor, rewritten by the host before each batch of steps.
@@ -240,9 +244,9 @@ __global__ __launch_bounds__(256) void k_place_neurons(float* V, int32_t* last_f
 }
 
 // Slot index of in-slot k of lane l in a slice whose slots start at slot_off
-// (chunk-major: 4 slots per lane per chunk).
-__device__ __forceinline__ uint32_t nc_slot_of(uint32_t slot_off, uint32_t l, uint32_t k) {
-  return slot_off + (k >> 2) * 256 + l * 4 + (k & 3);
+// (chunk-major: NC_CHUNK slots per lane per chunk).
+NC_HD uint32_t nc_slot_of(uint32_t slot_off, uint32_t l, uint32_t k) {
+  return slot_off + (k >> NC_CHUNK_LOG) * (64u * NC_CHUNK) + l * NC_CHUNK + (k & (NC_CHUNK - 1u));
 }
 
 // Group spike trains of a batch of steps: one Philox draw per (step, group),
@@ -503,8 +507,8 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
       if (t0 && t < pend_n) e = pl[t];
       const bool arrive = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
       const uint32_t rel = e >> 8;
-      const uint32_t tl = (rel & 255u) >> 2;
-      const uint32_t k = ((rel >> 8) << 2) | (rel & 3u);
+      const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
+      const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
       const uint32_t Gt = (uint32_t)__shfl((int)G, (int)tl);
       if (arrive) {
         __hip_atomic_fetch_or(a.amask + rec.mask_off + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
