@@ -285,9 +285,10 @@ nc_status nc_set_counting(nc_sim* sim, int enable);
  * launches; a chunk is replayed while the set of networks is unchanged);
  * stage_tables: copy the STDP trace tables into LDS per workgroup (pays when
  * many synapses see STDP per step) instead of reading them through the
- * caches; fanout_waves: the fan-out kernel's waves, each scanning every
- * fanout_waves-th slice for fired neurons (never more than there are
- * slices).  Results do not depend on the launch shape.  Default
+ * caches; fanout_waves: the most waves of the fan-out kernel, which share
+ * the step's records of slices with fired neurons (below the cap, one
+ * record per wave; never more waves than records can need).  Results do
+ * not depend on the launch shape.  Default
  * {4, 0, 0, 0, 0}; nc_sim_tune keeps fanout_waves as it finds it. */
 typedef struct nc_launch_cfg {
   uint32_t waves_per_block;
