@@ -425,7 +425,8 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   const bool fire = valid && ((((NC_THRESHOLD < V) || f_bg) && refr_ok) || f_in);
   if (fire) {
     lf = s;
-    a.firings[pos] = a.firings[pos] + 1u;
+    // an atomic whose result is unused: the wave does not wait for the count
+    __hip_atomic_fetch_add(&a.firings[pos], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
   // AP waveform (src/NeuCor.cpp:706-714) from the host table.
   const int32_t dlf = s - lf;
