@@ -9,15 +9,16 @@
 //             wave applies post-side STDP to the incoming synapses of each
 //             of its fired lanes, and turns the slice's spikes arriving at s
 //             (the pending list of s's window) into active-slot bits
-//   k_fanout  a capped grid of waves, each taking every nwaves-th slice's
-//             fired record: Synapse::fire on the fired lanes' outgoing
-//             synapses (source-order latch copy, out->in slot map),
-//             appending accepted spikes to the target slice's pending list
-//             of their arrival's window
+//   k_fanout  waves sharing the step's records of slices with fired lanes
+//             (64 lists, one record per wave below a cap): Synapse::fire on
+//             the fired lanes' outgoing synapses (source-order latch copy,
+//             out->in slot map), appending accepted spikes to the target
+//             slice's pending list of their arrival's window
 // The kernel boundary orders the phases.  Every synapse slot has one writer
 // per phase (its target's wave in k_neuron, its source's wave in k_fanout);
-// the only atomics are the pending-list append (per target slice) and the
-// arrival bits (workgroup scope, inside one wave's own slice).
+// the atomics are the appends to the pending lists (per target slice) and
+// to the fired lists (per list), the firing count, and the active bits
+// (workgroup scope, inside one wave's own slice).
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -548,9 +549,9 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
 }
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
-// fired neuron: each wave of the grid scans every nwaves-th fired record of
-// the active slices, 64 per load, and takes the slices with fired lanes one
-// by one; one writer per synapse (its source), latch permitting.  Accepted spikes latch
+// fired neuron: each wave takes every (nwaves / 64)-th record of one of the
+// step's 64 fired-slice lists, 64 per load, and the slices one by one; one
+// writer per synapse (its source), latch permitting.  Accepted spikes latch
 // depolarisation and arrival step into the target slot and join the target
 // slice's pending list.
 __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, const nc_edge& ed, int32_t s,
