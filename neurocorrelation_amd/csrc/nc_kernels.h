@@ -30,7 +30,6 @@
 #define NC_SLICE_NONE 0xFFFFFFFFu
 #define NC_LDS_TAB_BYTES 32768  // trace tables up to this size are staged in LDS
 #define NC_WAP_K 8       // AP-table entries passed as kernel arguments
-#define NC_MW 4          // active-mask words (32 slots each) a lane holds per pass
 // Spikes in flight to a slice sit in one of two pending lists by arrival
 // window: list (arrival >> NC_PEND_LOG_W) & 1, each at the slice's slot range
 // of its own array (at most one spike in flight per slot).  An entry is
