@@ -558,13 +558,14 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   // a spike still in flight or in its release window (:728)
   if (ed.arr + a.r_rel > s) return;
   const int32_t arr = s + (int32_t)(ed.tq_d & 0xffu);
+  const uint32_t tq = ed.tq_d >> 8;
+  const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
+  // the list place is taken before the slot is read, so that the two wait together
+  const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
   const nc_slot r = a.slots[ed.j];
   a.slots[ed.j].depol = a.depol_k * r.w;
   a.slots[ed.j].arr_inh = arr * 2 + (r.arr_inh & 1);
   a.edges[e].arr = arr;
-  const uint32_t tq = ed.tq_d >> 8;
-  const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
-  const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
   (plist ? a.pend1 : a.pend0)[ed.tbase + at] = ((ed.j - ed.tbase) << 8) | ((uint32_t)arr & 255u);
   ++n_acc;
 }
