@@ -423,11 +423,7 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   // charge_thresholdCheck (src/NeuCor.cpp:682-686) + input fires (:326-331).
   const bool refr_ok = (s - lf) > a.d_ref;
   const bool fire = valid && ((((NC_THRESHOLD < V) || f_bg) && refr_ok) || f_in);
-  if (fire) {
-    lf = s;
-    // an atomic whose result is unused: the wave does not wait for the count
-    __hip_atomic_fetch_add(&a.firings[pos], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (fire) lf = s;
   // AP waveform (src/NeuCor.cpp:706-714) from the host table.
   const int32_t dlf = s - lf;
   if (dlf <= a.d_ref) {
@@ -444,7 +440,11 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   if (valid) {
     // a neuron at rest stays at the base level bit for bit: no store
     if (__float_as_uint(V) != V_in) a.V[pos] = V;
-    if (fire) a.last_fire[pos] = lf;
+    if (fire) {
+      a.last_fire[pos] = lf;
+      // an atomic whose result is unused: the wave does not wait for the count
+      __hip_atomic_fetch_add(&a.firings[pos], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
   }
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
