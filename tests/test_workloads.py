@@ -1,6 +1,7 @@
 """CPU: the workload generators behind bench.py (sizes, degrees, step budgets
-and drives of the batched and served specs; deterministic per seed), and the
-benchmark's algorithmic byte model of the neuron-step kernel."""
+and drives of the batched and served specs; deterministic per seed), the
+benchmark's algorithmic byte model of the neuron-step kernel, and its refusal
+of more than one GPU."""
 import importlib.util
 import math
 import os
@@ -62,3 +63,16 @@ def test_neuron_kernel_byte_model():
     expect = (20 * 640 + 40 * 10 + 4 * (3 + 2) + 16 * 10 + 16 * 1 + 20 * 5 + 12 * 0.5 + 4 * 20)
     assert abs(b - expect) < 1e-9
     assert ev["A"] == 10 and ev["F_in"] == 5
+
+
+def test_bench_measures_one_gpu_only():
+    """bench.py --gpus N with N > 1 (or a launcher's WORLD_SIZE > 1) stops
+    with a message, before touching a GPU."""
+    import subprocess
+    import sys
+    for extra_env, args in (({}, ["--gpus", "2"]), ({"WORLD_SIZE": "2"}, [])):
+        env = dict(os.environ, **extra_env)
+        p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py")] + args, env=env,
+                           capture_output=True, text=True, timeout=120)
+        assert p.returncode == 2, p.stderr
+        assert "one GPU" in p.stderr and p.stdout == ""
