@@ -437,15 +437,6 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
     }
   }
 
-  if (valid) {
-    // a neuron at rest stays at the base level bit for bit: no store
-    if (__float_as_uint(V) != V_in) a.V[pos] = V;
-    if (fire) {
-      a.last_fire[pos] = lf;
-      // an atomic whose result is unused: the wave does not wait for the count
-      __hip_atomic_fetch_add(&a.firings[pos], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
-  }
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
   {
@@ -528,6 +519,17 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
   const uint32_t sm_new = sh_sum[wl][lane];
   if (live && sm_new != sm) a.asum[pos] = sm_new;
+  // The neuron's own state is written last (nothing in this kernel reads it
+  // back), so that no wait on the way above covers these stores.
+  if (valid) {
+    // a neuron at rest stays at the base level bit for bit: no store
+    if (__float_as_uint(V) != V_in) a.V[pos] = V;
+    if (fire) {
+      a.last_fire[pos] = lf;
+      // an atomic whose result is unused: the wave does not wait for the count
+      __hip_atomic_fetch_add(&a.firings[pos], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
 
   if (a.raster_cap) {
     const uint32_t rt = nc_wave_append(fire, a.raster_cnt);
