@@ -612,7 +612,7 @@ hipEvent_t get_event(nc_sim* sim, size_t i) {
 
 // ---------------------------------------------------------------------------
 
-extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64_t* bytes) {
+extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64_t* bytes) try {
   if (!bytes) return fail(NC_ERR_ARG, "null argument");
   // the buffers nc_sim_create allocates for one network of this size
   const uint64_t P = (n_neurons + 63) / 64 * 64 + 64, Q = P / 64;
@@ -625,8 +625,9 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
            + (Q + 2) * NC_NCTR * 8;                        // counters
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, uint64_t* slots) {
+extern "C" nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, uint64_t* slots) try {
   if (!net) return fail(NC_ERR_ARG, "null network");
   if (net->image) {
     if (positions) *positions = (uint64_t)net->image->nq * 64;
@@ -639,6 +640,7 @@ extern "C" nc_status nc_net_device_size(const nc_net* net, uint64_t* positions, 
   if (slots) *slots = L.nslots;
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_net_prepare(nc_net* net) {
   if (!net) return fail(NC_ERR_ARG, "null network");
@@ -651,7 +653,7 @@ extern "C" nc_status nc_net_prepare(nc_net* net) {
   return NC_OK;
 }
 
-extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
+extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
   if (!spec || !out) return fail(NC_ERR_ARG, "null argument");
   *out = nullptr;
   int ndev = 0;
@@ -661,6 +663,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
   if (spec->max_synapses >= (1ull << 32) - (1ull << 24)) return fail(NC_ERR_ARG, "max_synapses too large");
   nc_sim* sim = new (std::nothrow) nc_sim();
   if (!sim) return fail(NC_ERR_OOM, "host allocation failed");
+  // destroys the half-built simulation on a failure or an exception
+  std::unique_ptr<nc_sim, void (*)(nc_sim*)> guard(sim, nc_sim_destroy);
   sim->device = spec->device;
   nc_status st = NC_OK;
   auto H = [&](hipError_t e, const char* what) {
@@ -735,13 +739,11 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) {
   sim->a_slots.init(sim->max_slots / 256);
   sim->a_out.init(sim->max_out);
   sim->a_masks.init(sim->max_mwords / 64);
-  if (st != NC_OK) {
-    nc_sim_destroy(sim);
-    return st;
-  }
-  *out = sim;
+  if (st != NC_OK) return st;
+  *out = guard.release();
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (!sim) return;
@@ -761,7 +763,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   delete sim;
 }
 
-extern "C" nc_status nc_sim_free_bytes(const nc_sim* sim, uint64_t* free_bytes) {
+extern "C" nc_status nc_sim_free_bytes(const nc_sim* sim, uint64_t* free_bytes) try {
   if (!sim || !free_bytes) return fail(NC_ERR_ARG, "null argument");
   NC_HIP(hipSetDevice(sim->device));
   size_t fr = 0, tot = 0;
@@ -769,10 +771,11 @@ extern "C" nc_status nc_sim_free_bytes(const nc_sim* sim, uint64_t* free_bytes) 
   *free_bytes = fr;
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params* params,
                                 const nc_drive_spec* drive, uint32_t steps_to_run,
-                                int32_t* net_id) {
+                                int32_t* net_id) try {
   if (!sim || !net || !params || !net_id) return fail(NC_ERR_ARG, "null argument");
   NC_HIP(hipSetDevice(sim->device));
   // The drive tables below are host vectors a pending upload may still read.
@@ -991,8 +994,9 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   *net_id = id;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) {
+extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) try {
   NetRec* r;
   nc_status stt = check_net(sim, net_id, &r);
   if (stt != NC_OK) return stt;
@@ -1020,8 +1024,9 @@ extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) {
   sim->active_dirty = true;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
+extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   if (n_steps == 0) return NC_OK;
   NC_HIP(hipSetDevice(sim->device));
@@ -1253,9 +1258,10 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) {
   }
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_sim_progress(nc_sim* sim, int32_t net_id, uint32_t* steps_done,
-                                     uint32_t* steps_left) {
+                                     uint32_t* steps_left) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1263,8 +1269,9 @@ extern "C" nc_status nc_sim_progress(nc_sim* sim, int32_t net_id, uint32_t* step
   if (steps_left) *steps_left = r->steps_left;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_get_time(nc_sim* sim, int32_t net_id, float* time_ms) {
+extern "C" nc_status nc_get_time(nc_sim* sim, int32_t net_id, float* time_ms) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1272,9 +1279,10 @@ extern "C" nc_status nc_get_time(nc_sim* sim, int32_t net_id, float* time_ms) {
   *time_ms = (float)r->steps_done * sim->T.dt;
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_set_params(nc_sim* sim, int32_t net_id, float learning_rate,
-                                   float pre_factor, float post_factor) {
+                                   float pre_factor, float post_factor) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1286,10 +1294,11 @@ extern "C" nc_status nc_set_params(nc_sim* sim, int32_t net_id, float learning_r
   r->post_f = post_factor;
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_events,
                                     const uint32_t* step_off, const uint32_t* ids,
-                                    const uint8_t* kind) {
+                                    const uint8_t* kind) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1303,6 +1312,7 @@ extern "C" nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_even
     r->events.push_back({r->steps_done + step_off[k], {r->pos(ids[k]), kind[k]}});
   return NC_OK;
 }
+NC_ABI_CATCH
 
 namespace {
 template <class T>
@@ -1330,31 +1340,34 @@ nc_status read_slots(nc_sim* sim, NetRec& r, std::vector<nc_slot>* hs) {
 }
 }  // namespace
 
-extern "C" nc_status nc_read_potential(nc_sim* sim, int32_t net_id, float* out, uint32_t n) {
+extern "C" nc_status nc_read_potential(nc_sim* sim, int32_t net_id, float* out, uint32_t n) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
   NC_HIP(hipSetDevice(sim->device));
   return read_pos_array(sim, *r, sim->V, out, n);
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_read_last_fire(nc_sim* sim, int32_t net_id, int32_t* out, uint32_t n) {
+extern "C" nc_status nc_read_last_fire(nc_sim* sim, int32_t net_id, int32_t* out, uint32_t n) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
   NC_HIP(hipSetDevice(sim->device));
   return read_pos_array(sim, *r, sim->last_fire, out, n);
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_read_firings(nc_sim* sim, int32_t net_id, uint32_t* out, uint32_t n) {
+extern "C" nc_status nc_read_firings(nc_sim* sim, int32_t net_id, uint32_t* out, uint32_t n) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
   NC_HIP(hipSetDevice(sim->device));
   return read_pos_array(sim, *r, sim->firings, out, n);
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, uint64_t n) {
+extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, uint64_t n) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1375,8 +1388,9 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   for (uint64_t k = 0; k < n; ++k) out[k] = w[r->maps->slot_of_syn[k]];
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* w, uint64_t n) {
+extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* w, uint64_t n) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1396,8 +1410,9 @@ extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* 
   NC_HIP(hipStreamSynchronize(sim->stream));
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, const uint64_t* syn, const float* w) {
+extern "C" nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, const uint64_t* syn, const float* w) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1417,9 +1432,10 @@ extern "C" nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, con
   NC_HIP(hipStreamSynchronize(sim->stream));
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t* arrival,
-                                           int32_t* last_arrival, float* depol, uint64_t n) {
+                                           int32_t* last_arrival, float* depol, uint64_t n) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1439,9 +1455,10 @@ extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t*
   }
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_read_raster(nc_sim* sim, int32_t net_id, uint32_t* steps, uint32_t* ids,
-                                    uint64_t cap, uint64_t* n_out) {
+                                    uint64_t cap, uint64_t* n_out) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1479,6 +1496,7 @@ extern "C" nc_status nc_read_raster(nc_sim* sim, int32_t net_id, uint32_t* steps
   NC_HIP(hipMemcpy(sim->raster_cnt, &nrest, 4, hipMemcpyHostToDevice));
   return NC_OK;
 }
+NC_ABI_CATCH
 
 // ---------------------------------------------------------------------------
 // Checkpoints: a network's dynamic state in network-local order.
@@ -1500,7 +1518,7 @@ struct nc_ckpt_header {
 };
 }  // namespace
 
-extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_t cap, uint64_t* size) {
+extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_t cap, uint64_t* size) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1549,8 +1567,9 @@ extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_
   }
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf, uint64_t size) {
+extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf, uint64_t size) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1654,8 +1673,9 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   r->events = events;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_reset_activities(nc_sim* sim, int32_t net_id) {
+extern "C" nc_status nc_reset_activities(nc_sim* sim, int32_t net_id) try {
   NetRec* r;
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
@@ -1664,8 +1684,9 @@ extern "C" nc_status nc_reset_activities(nc_sim* sim, int32_t net_id) {
   NC_HIP(hipStreamSynchronize(sim->stream));
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) {
+extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) try {
   if (!sim || (!out && n)) return fail(NC_ERR_ARG, "null argument");
   if (n > NC_NUM_COUNTERS) return fail(NC_ERR_RANGE, "at most NC_NUM_COUNTERS counters");
   NC_HIP(hipSetDevice(sim->device));
@@ -1684,8 +1705,9 @@ extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) {
   for (uint32_t k = 0; k < n; ++k) out[k] = all[k];
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) {
+extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) try {
   if (!sim || !cfg) return fail(NC_ERR_ARG, "null argument");
   const uint32_t w = cfg->waves_per_block;
   if (w != 1 && w != 2 && w != 4 && w != 8) return fail(NC_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
@@ -1697,14 +1719,16 @@ extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) {
   sim->cfg = *cfg;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg) {
+extern "C" nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg) try {
   if (!sim || !cfg) return fail(NC_ERR_ARG, "null argument");
   *cfg = sim->cfg;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen) {
+extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen) try {
   if (!sim || steps_per_trial == 0) return fail(NC_ERR_ARG, "null sim or zero steps");
   NC_HIP(hipSetDevice(sim->device));
   hipEvent_t e0 = nullptr, e1 = nullptr;
@@ -1748,21 +1772,24 @@ extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launc
   if (chosen) *chosen = sim->cfg;
   return st;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_set_counting(nc_sim* sim, int enable) {
+extern "C" nc_status nc_set_counting(nc_sim* sim, int enable) try {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   sim->counting = enable != 0;
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_synchronize(nc_sim* sim) {
+extern "C" nc_status nc_sim_synchronize(nc_sim* sim) try {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   NC_HIP(hipSetDevice(sim->device));
   NC_HIP(hipStreamSynchronize(sim->stream));
   return NC_OK;
 }
+NC_ABI_CATCH
 
-extern "C" nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask, uint32_t every) {
+extern "C" nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask, uint32_t every) try {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   sim->kernel_timing = kernel_mask & 0xF;
   sim->timing_every = every ? every : 1;
@@ -1773,12 +1800,14 @@ extern "C" nc_status nc_set_kernel_timing(nc_sim* sim, int kernel_mask, uint32_t
   }
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_read_kernel_timing(nc_sim* sim, uint32_t kernel, uint64_t* launches,
-                                           double* total_ms) {
+                                           double* total_ms) try {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   if (kernel > 3) return fail(NC_ERR_RANGE, "kernel index out of range");
   if (launches) *launches = sim->k_launch[kernel];
   if (total_ms) *total_ms = sim->k_ms[kernel];
   return NC_OK;
 }
+NC_ABI_CATCH

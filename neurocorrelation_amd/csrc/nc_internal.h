@@ -1,7 +1,9 @@
 // Host-side internals shared by the network builder and the engine.
 #pragma once
 #include <stdint.h>
+#include <exception>
 #include <memory>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -35,6 +37,16 @@ struct nc_net {
 namespace nc {
 void set_error(const std::string& msg);
 nc_status fail(nc_status code, const std::string& msg);
+
+// Handlers of the function-try-blocks around the C entry points: no C++
+// exception crosses the C ABI; a host allocation failure is NC_ERR_OOM.
+#define NC_ABI_CATCH                                                              \
+  catch (const std::bad_alloc&) {                                                 \
+    return nc::fail(NC_ERR_OOM, "host allocation failed");                        \
+  }                                                                               \
+  catch (const std::exception& e) {                                               \
+    return nc::fail(NC_ERR_STATE, std::string("unexpected host error: ") + e.what()); \
+  }
 void finish_net_stats(nc_net* net);
 nc_status check_wiring_spec(const nc_wiring_spec& s);
 

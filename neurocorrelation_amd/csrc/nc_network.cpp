@@ -290,7 +290,7 @@ extern "C" nc_status nc_net_from_edges(uint32_t n, const float* xyz, uint64_t n_
 }
 
 extern "C" nc_status nc_net_info(const nc_net* net, uint32_t* n, uint64_t* n_syn,
-                                 uint32_t* max_in_degree, uint32_t* max_delay) {
+                                 uint32_t* max_in_degree, uint32_t* max_delay) try {
   if (!net) return fail(NC_ERR_ARG, "null network");
   if (n) *n = net->n;
   if (n_syn) *n_syn = net->n_syn();
@@ -298,9 +298,10 @@ extern "C" nc_status nc_net_info(const nc_net* net, uint32_t* n, uint64_t* n_syn
   if (max_delay) *max_delay = net->max_delay;
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" nc_status nc_net_export(const nc_net* net, float* xyz, int64_t* in_ptr, uint32_t* in_src,
-                                   float* w, uint8_t* delay_steps, uint8_t* inhibitory) {
+                                   float* w, uint8_t* delay_steps, uint8_t* inhibitory) try {
   if (!net) return fail(NC_ERR_ARG, "null network");
   const uint64_t S = net->n_syn();
   if (xyz) memcpy(xyz, net->xyz.data(), 12ull * net->n);
@@ -311,11 +312,12 @@ extern "C" nc_status nc_net_export(const nc_net* net, float* xyz, int64_t* in_pt
   if (inhibitory) memcpy(inhibitory, net->inh.data(), S);
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" void nc_net_free(nc_net* net) { delete net; }
 
 extern "C" nc_status nc_net_select_inputs(const nc_net* net, const nc_drive_spec* drive,
-                                          uint32_t* ids, uint32_t* groups) {
+                                          uint32_t* ids, uint32_t* groups) try {
   if (!net || !drive) return fail(NC_ERR_ARG, "null argument");
   nc::InputSel sel;
   nc_status st = nc::select_inputs(*net, *drive, &sel);
@@ -324,5 +326,6 @@ extern "C" nc_status nc_net_select_inputs(const nc_net* net, const nc_drive_spec
   if (groups) memcpy(groups, sel.group.data(), 4 * sel.group.size());
   return NC_OK;
 }
+NC_ABI_CATCH
 
 extern "C" const char* nc_last_error(void) { return nc::g_error.c_str(); }
