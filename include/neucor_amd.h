@@ -29,6 +29,8 @@
 extern "C" {
 #endif
 
+/* Every entry point reports failure through its status (the message through
+ * nc_last_error); none lets a C++ exception escape. */
 typedef enum nc_status {
   NC_OK = 0,
   NC_ERR_ARG = 1,       /* bad argument (std::invalid_argument-like)            */
@@ -36,7 +38,8 @@ typedef enum nc_status {
   NC_ERR_HIP = 3,       /* a HIP runtime call failed                             */
   NC_ERR_OOM = 4,       /* device or host allocation failed                      */
   NC_ERR_CAPACITY = 5,  /* the nc_sim has no room for the network (admission)    */
-  NC_ERR_STATE = 6,     /* call not valid in the current state                   */
+  NC_ERR_STATE = 6,     /* call not valid in the current state, or an unexpected */
+                        /* host-side error                                       */
   NC_ERR_NO_DEVICE = 7  /* no usable GPU                                          */
 } nc_status;
 
