@@ -29,8 +29,6 @@
 extern "C" {
 #endif
 
-/* Every entry point reports failure through its status (the message through
- * nc_last_error); none lets a C++ exception escape. */
 typedef enum nc_status {
   NC_OK = 0,
   NC_ERR_ARG = 1,       /* bad argument (std::invalid_argument-like)            */
