@@ -1,7 +1,8 @@
 """CPU: the served form's request loop (admission by room, retirement at the
-job's budget, lanes alternating on the GPU) over a stand-in simulation that
-counts steps and room the way nc_sim does, so the scheduling is checked
-without a device.  The GPU tests check the results themselves."""
+job's budget, lanes alternating on the GPU, refusals other than "no room"
+raised) over a stand-in simulation that counts steps and room the way nc_sim
+does, so the scheduling is checked without a device.  The GPU tests check
+the results themselves."""
 import hashlib
 
 import numpy as np
@@ -119,3 +120,21 @@ def test_oversized_job_is_refused_and_the_rest_served():
     assert "error" in by[1].result
     for i in (0, 2, 3):
         assert by[i].result == expected(by[i])
+
+
+def test_errors_other_than_room_stop_the_loop():
+    """Only NC_ERR_CAPACITY makes a job wait; any other refusal is raised."""
+    specs = SV.served_specs(2, seed=7, n_range=(300, 300), k_range=(20, 20), steps_range=(40, 40))
+    jobs = [SV.Job(i, sp, 0.0) for i, sp in enumerate(specs)]
+    dev = Device()
+
+    class Refusing(FakeSim):
+        def add(self, net, params, drive, steps=0):
+            raise NCError(1, "bad argument")
+
+    server = SV.Server(max_neurons=4000, max_synapses=100_000, max_nets=4, chunk=20, builders=2, lanes=2,
+                       sim_factory=lambda *a, **k: Refusing(dev, *a, **k))
+    with pytest.raises(NCError) as e:
+        server.run(jobs, t_limit=30)
+    server.close()
+    assert e.value.code == 1
