@@ -1323,7 +1323,6 @@ nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t 
   NC_HIP(hipMemcpyAsync(tmp.data(), dev + (size_t)r.q0 * 64, tmp.size() * sizeof(T), hipMemcpyDeviceToHost,
                         sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
-  const uint32_t p0 = r.q0 * 64;
   for (uint32_t i = 0; i < n; ++i) out[i] = tmp[r.maps->pos_of_id[i]];
   return NC_OK;
 }
