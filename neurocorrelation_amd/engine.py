@@ -8,7 +8,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import DriveSpec, LaunchCfg, Params, SimSpec, WiringSpec, check, lib, ptr
+from ._lib import LaunchCfg, Params, SimSpec, WiringSpec, check, lib, ptr
 
 NEVER = _lib.NC_NEVER_STEP
 

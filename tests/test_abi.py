@@ -5,7 +5,6 @@ import os
 import re
 
 import numpy as np
-import pytest
 
 import neurocorrelation_amd as nc
 from neurocorrelation_amd import _lib

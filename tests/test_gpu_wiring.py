@@ -2,7 +2,6 @@
 the host builder's network bit for bit (positions, rows in ascending source
 id, delays, weights, sign classes); the host builder is pinned to the
 oracle's brute-force restatement in test_wiring.py."""
-import math
 import sys
 import time
 

@@ -3,7 +3,6 @@ job's budget, lanes alternating on the GPU, refusals other than "no room"
 raised) over a stand-in simulation that counts steps and room the way nc_sim
 does, so the scheduling is checked without a device.  The GPU tests check
 the results themselves."""
-import hashlib
 
 import numpy as np
 import pytest

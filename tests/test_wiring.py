@@ -1,6 +1,5 @@
 """The engine's network builder (spatial hash) against the oracle's brute
 force restatement of the wiring rule, and its invariants."""
-import math
 
 import numpy as np
 import pytest
