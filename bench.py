@@ -181,15 +181,24 @@ def bench_batched(args):
     print("[bench] setup %.1fs" % setup_s, file=sys.stderr, flush=True)
 
     sim.synchronize()
-    t_start = time.perf_counter()
+    tune_table, tune_s = None, 0.0
     if args.tune_steps:
-        chosen = sim.tune(args.tune_steps)
-        done = 2 * 32 * args.tune_steps
-        print("[bench] tuned launch %s after %d steps (%.2fs)" % (chosen, done, time.perf_counter() - t_start),
-              file=sys.stderr, flush=True)
+        # the engine's 32 candidate shapes, each timed from the same saved
+        # state over 3 rounds (tuning takes no simulated step)
+        t_tune = time.perf_counter()
+        cands = [dict(waves_per_block=w, xcd_swizzle=x & 1, graph_steps=g, stage_tables=x >> 1)
+                 for g in (0, min(args.tune_steps, 1024)) for w in (1, 2, 4, 8) for x in range(4)]
+        chosen, ms = sim.tune_list(cands, steps_per_trial=args.tune_steps, rounds=3)
+        tune_s = time.perf_counter() - t_tune
+        tune_table = [dict(c, median_ms=float(m)) for c, m in zip(cands, ms)]
+        for row in tune_table:
+            print("[bench] tune %s" % row, file=sys.stderr, flush=True)
+        print("[bench] tuned launch %s (%.2fs)" % (chosen, tune_s), file=sys.stderr, flush=True)
     else:
         sim.set_launch(args.waves_per_block)
-        chosen, done = sim.launch(), 0
+        chosen = sim.launch()
+    done = 0
+    t_start = time.perf_counter()
     while done < steps:
         k = min(args.chunk, steps - done)
         sim.step(k)
@@ -237,6 +246,7 @@ def bench_batched(args):
                    "synapses": S_tot, "steps_per_network": steps, "launch": chosen,
                    "tune_steps_per_trial": args.tune_steps},
         "synapse_updates_per_s": S_tot * steps / elapsed,
+        "tune_s": tune_s, "sims_per_s_with_tuning": len(built) / (elapsed + tune_s), "tune_table": tune_table,
         "sequential_plain": {"sample": len(sample), "mean_s_per_sim": seq_per_sim,
                              "sims_per_s": 1.0 / seq_per_sim if seq_per_sim else None, "identical_final_state": True},
         "speedup_vs_sequential": sims_s * seq_per_sim if seq_per_sim else None,

@@ -194,7 +194,11 @@ nc_status nc_sim_remove(nc_sim* sim, int32_t net_id);
 
 /* Replaces NeuCor::run() (src/NeuCor.cpp:583-617) with runAll = true and
  * runSpeed = dt: advance every active network by up to n_steps steps
- * (networks with fewer steps left stop early). */
+ * (networks with fewer steps left stop early).  Launches kernels and copies
+ * the per-call network windows and due host events to buffers sized by
+ * nc_sim_create, nc_sim_add and nc_push_forced: it never allocates device
+ * memory (under a graph launch shape it captures and instantiates a graph
+ * when the shape or the set of networks changed). */
 nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps);
 
 /* Steps each network has completed and has left (0 = finished when it was
@@ -215,7 +219,9 @@ nc_status nc_set_params(nc_sim* sim, int32_t net_id, float learning_rate,
  * :604-607, push onto the simulation queue).  Event k fires neuron ids[k]
  * at step offset step_off[k] (1 = the next step) of net_id; kind[k] is 0 for
  * an input fire (Neuron::fire, bypasses refractory) and 1 for a background
- * fire (scheduledFireTime, respects refractory).  Consumed by nc_sim_step. */
+ * fire (scheduledFireTime, respects refractory).  Consumed by nc_sim_step.
+ * Grows the device event buffer to hold every queued event when needed
+ * (the step itself never allocates); on failure no event is queued. */
 nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_events,
                          const uint32_t* step_off, const uint32_t* ids,
                          const uint8_t* kind);
@@ -283,7 +289,8 @@ nc_status nc_set_counting(nc_sim* sim, int enable);
  * nc_sim.  waves_per_block: 1, 2, 4 or 8 slices (64 neurons each) per
  * workgroup; xcd_swizzle: each of the 8 XCDs takes one contiguous range of
  * slices; graph_steps: steps per captured hipGraph chunk (0 = direct
- * launches; a chunk is replayed while the set of networks is unchanged);
+ * launches, at most 1024; a chunk is replayed while the set of networks is
+ * unchanged);
  * stage_tables: copy the STDP trace tables into LDS per workgroup (pays when
  * many synapses see STDP per step) instead of reading them through the
  * caches; fanout_waves: the most waves of the fan-out kernel, which share
@@ -301,11 +308,25 @@ typedef struct nc_launch_cfg {
 nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg);
 nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg);
 
-/* Autotuner of the launch shape: runs steps_per_trial ordinary steps of every
- * active network under each candidate shape (after a warm-up of the same
- * length, which also captures graphs), keeps the fastest by device time and
- * reports it in *chosen.  The steps count as simulated steps. */
+/* Autotuner of the launch shape over the 32 shapes {1, 2, 4, 8 slices per
+ * workgroup} x {XCD order off/on} x {LDS tables off/on} x {no graph, graph
+ * chunks of steps_per_trial steps}, with the current fanout_waves: as
+ * nc_sim_tune_list with 3 rounds. */
 nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen);
+
+/* Autotuner over the caller's candidate shapes.  The simulation's state is
+ * saved on the device first (the call allocates a copy of its buffers); every
+ * trial starts from that state -- a warm-up trial of steps_per_trial steps,
+ * which also captures the candidate's graph, the state put back, then the
+ * timed trial of the same steps -- so that every candidate times the same
+ * work.  `rounds` rounds visit the candidates in rotated order; each
+ * candidate's median device time is its score, written to ms_out[i] (may be
+ * NULL).  The fastest becomes the launch shape and is reported in *chosen.
+ * Afterwards the simulation is exactly where it was before the call (no
+ * simulated step is taken). */
+nc_status nc_sim_tune_list(nc_sim* sim, const nc_launch_cfg* cands, uint32_t n_cands,
+                           uint32_t steps_per_trial, uint32_t rounds, nc_launch_cfg* chosen,
+                           float* ms_out);
 
 /* Block until every step enqueued on the nc_sim's stream has finished. */
 nc_status nc_sim_synchronize(nc_sim* sim);

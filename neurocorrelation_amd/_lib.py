@@ -102,6 +102,8 @@ SIGNATURES = {
     "nc_sim_set_launch": (C.c_int, [P, C.POINTER(LaunchCfg)]),
     "nc_sim_get_launch": (C.c_int, [P, C.POINTER(LaunchCfg)]),
     "nc_sim_tune": (C.c_int, [P, C.c_uint32, C.POINTER(LaunchCfg)]),
+    "nc_sim_tune_list": (C.c_int, [P, C.POINTER(LaunchCfg), C.c_uint32, C.c_uint32, C.c_uint32,
+                                   C.POINTER(LaunchCfg), C.POINTER(C.c_float)]),
     "nc_sim_synchronize": (C.c_int, [P]),
     "nc_set_kernel_timing": (C.c_int, [P, C.c_int, C.c_uint32]),
     "nc_read_kernel_timing": (C.c_int, [P, C.c_uint32, u64p, C.POINTER(C.c_double)]),

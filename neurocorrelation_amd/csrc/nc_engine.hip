@@ -431,7 +431,7 @@ struct nc_sim {
   RangeAlloc a_groups, a_thr;
   std::vector<nc_slice_rec> active_host;
   std::vector<uint32_t> slice_off_host, slice_moff_host;
-  bool active_dirty = true, groups_dirty = true;
+  bool active_dirty = true;
   std::vector<uint32_t> slice_net_host;
   uint64_t raster_read = 0;
   bool act_identity = false;  // active record i is slice act_q0 + i
@@ -597,6 +597,22 @@ nc_status flush_net(nc_sim* sim, NetRec& r) {
                      (int32_t)r.steps_done, r.lr, r.pre_f, r.post_f);
   NC_HIP(hipGetLastError());
   return NC_OK;
+}
+
+// Device room for every host event queued on the simulation's networks, so
+// that nc_sim_step, which uploads those due in its steps, never allocates.
+nc_status reserve_events(nc_sim* sim) {
+  size_t total = 0;
+  for (const NetRec& r : sim->nets)
+    if (r.live) total += r.events.size();
+  if (total <= sim->d_ev_pos.n && total <= sim->d_ev_kind.n) return NC_OK;
+  if (sim->upload_pending) {
+    NC_HIP(hipEventSynchronize(sim->upload_done));
+    sim->upload_pending = false;
+  }
+  nc_status st = grow(&sim->d_ev_pos, total, sim->stream);
+  if (st == NC_OK) st = grow(&sim->d_ev_kind, total, sim->stream);
+  return st;
 }
 
 hipEvent_t get_event(nc_sim* sim, size_t i) {
@@ -841,6 +857,13 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   }
   sl0 *= 256;
   mk0 *= 64;
+  auto release_ranges = [&]() {
+    sim->a_slices.release(q0, I.nq);
+    sim->a_slots.release(sl0 / 256, I.nslots / 256);
+    sim->a_out.release(o0, S);
+    sim->a_masks.release(mk0 / 64, I.nmwords / 64);
+    sim->nets[id].live = false;
+  };
 
   NetRec& r = sim->nets[id];
   r = NetRec();
@@ -880,11 +903,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     nc::InputSel sel;
     nc_status stt = nc::select_inputs(*net, *drive, &sel);
     if (stt != NC_OK) {
-      sim->a_slices.release(q0, I.nq);
-      sim->a_slots.release(sl0 / 256, I.nslots / 256);
-      sim->a_out.release(o0, S);
-      sim->a_masks.release(mk0 / 64, I.nmwords / 64);
-      r.live = false;
+      release_ranges();
       return stt;
     }
     r.has_drive = true;
@@ -923,8 +942,25 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
     }
     for (uint32_t g = 0; g < sel.n_groups_total; ++g)
       sim->groups[g0 + g] = nc_group{(uint32_t)t0 + (g < ncorr ? 0u : T), T, g, (uint32_t)id};
-    if (grew || sim->d_groups.n < sim->groups.size() || sim->d_thr.n < sim->thr.size()) {
-      sim->groups_dirty = true;  // whole tables, at the next step
+    const size_t gw = (sim->groups.size() + 31) / 32;
+    if (grew || sim->d_groups.n < sim->groups.size() || sim->d_thr.n < sim->thr.size() ||
+        sim->d_gfire.n < (size_t)NC_GRING * gw) {
+      // The device tables and the ring of group spike trains follow the host
+      // tables' capacity, here: nc_sim_step never allocates.
+      nc_status sg = grow(&sim->d_groups, sim->groups.size(), sim->stream);
+      if (sg == NC_OK) sg = grow(&sim->d_thr, sim->thr.size(), sim->stream);
+      if (sg == NC_OK) sg = grow(&sim->d_gfire, (size_t)NC_GRING * gw, sim->stream);
+      if (sg != NC_OK) {
+        for (uint32_t g = 0; g < r.ngrp; ++g) sim->groups[g0 + g] = dead;
+        sim->a_groups.release(g0, r.ngrp);
+        sim->a_thr.release(t0, r.nthr);
+        release_ranges();
+        return sg;
+      }
+      NC_HIP(hipMemcpyAsync(sim->d_groups.p, sim->groups.data(), sim->groups.size() * sizeof(nc_group),
+                            hipMemcpyHostToDevice, sim->stream));
+      NC_HIP(hipMemcpyAsync(sim->d_thr.p, sim->thr.data(), sim->thr.size() * 4, hipMemcpyHostToDevice,
+                            sim->stream));
     } else if (r.ngrp) {
       NC_HIP(hipMemcpyAsync(sim->d_groups.p + g0, sim->groups.data() + g0, r.ngrp * sizeof(nc_group),
                             hipMemcpyHostToDevice, sim->stream));
@@ -954,7 +990,19 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   const uint64_t n_inh = I.slot_inh.size(), o_jq = (I.nslots + n_inh + 1) & ~1ull;
   {
     nc_status sg = grow(&sim->d_stage, o_jq + 2 * S + 2, st);
-    if (sg != NC_OK) return sg;
+    if (sg != NC_OK) {
+      if (r.has_drive) {
+        const nc_group dead{0u, 1u, 0u, sim->max_nets};
+        for (uint32_t g = 0; g < r.ngrp; ++g) sim->groups[r.grp0 + g] = dead;
+        if (r.ngrp)
+          NC_HIP(hipMemcpy(sim->d_groups.p + r.grp0, sim->groups.data() + r.grp0, r.ngrp * sizeof(nc_group),
+                           hipMemcpyHostToDevice));
+        sim->a_groups.release(r.grp0, r.ngrp);
+        sim->a_thr.release(r.thr0, r.nthr);
+      }
+      release_ranges();
+      return sg;
+    }
   }
   uint32_t* const stg = sim->d_stage.p;
   auto blocks = [](uint64_t n) { return dim3((uint32_t)std::min<uint64_t>((n + 255) / 256, 4096)); };
@@ -1013,7 +1061,7 @@ extern "C" nc_status nc_sim_remove(nc_sim* sim, int32_t net_id) try {
     // the groups go back to the table, pointing at the never-live network
     const nc_group dead{0u, 1u, 0u, sim->max_nets};
     for (uint32_t g = 0; g < r->ngrp; ++g) sim->groups[r->grp0 + g] = dead;
-    if (r->ngrp && !sim->groups_dirty)
+    if (r->ngrp)
       NC_HIP(hipMemcpy(sim->d_groups.p + r->grp0, sim->groups.data() + r->grp0, r->ngrp * sizeof(nc_group),
                        hipMemcpyHostToDevice));
     sim->a_groups.release(r->grp0, r->ngrp);
@@ -1031,18 +1079,6 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   if (n_steps == 0) return NC_OK;
   NC_HIP(hipSetDevice(sim->device));
   hipStream_t st = sim->stream;
-  if (sim->groups_dirty) {
-    if (!sim->groups.empty()) {
-      nc_status s1 = grow(&sim->d_groups, sim->groups.size(), st);
-      if (s1 != NC_OK) return s1;
-      s1 = grow(&sim->d_thr, sim->thr.size(), st);
-      if (s1 != NC_OK) return s1;
-      NC_HIP(hipMemcpyAsync(sim->d_groups.p, sim->groups.data(), sim->groups.size() * sizeof(nc_group),
-                            hipMemcpyHostToDevice, st));
-      NC_HIP(hipMemcpyAsync(sim->d_thr.p, sim->thr.data(), sim->thr.size() * 4, hipMemcpyHostToDevice, st));
-    }
-    sim->groups_dirty = false;
-  }
   // Per-network window of this call.
   if (sim->upload_pending) {
     NC_HIP(hipEventSynchronize(sim->upload_done));
@@ -1116,10 +1152,9 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
     }
   }
   if (!ev_pos.empty()) {
-    nc_status s1 = grow(&sim->d_ev_pos, ev_pos.size(), st);
-    if (s1 != NC_OK) return s1;
-    s1 = grow(&sim->d_ev_kind, ev_kind.size(), st);
-    if (s1 != NC_OK) return s1;
+    // nc_push_forced and nc_sim_restore reserve room for every queued event
+    if (sim->d_ev_pos.n < ev_pos.size() || sim->d_ev_kind.n < ev_kind.size())
+      return fail(NC_ERR_STATE, "event buffer smaller than the queued events");
     NC_HIP(hipMemcpyAsync(sim->d_ev_pos.p, ev_pos.data(), ev_pos.size() * 4, hipMemcpyHostToDevice, st));
     NC_HIP(hipMemcpyAsync(sim->d_ev_kind.p, ev_kind.data(), ev_kind.size(), hipMemcpyHostToDevice, st));
   }
@@ -1131,14 +1166,10 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   a.thr = sim->d_thr.p;
   a.n_groups = (uint32_t)sim->groups.size();
   a.gwords = (a.n_groups + 31) / 32;
-  if (a.n_groups) {
-    nc_status s1 = grow(&sim->d_gfire, (size_t)max_run * a.gwords, st);
-    if (s1 != NC_OK) return s1;
-    a.gfire = sim->d_gfire.p;
-    const uint64_t nthreads = (uint64_t)max_run * a.gwords * 32;
-    hipLaunchKernelGGL(k_groups, dim3((uint32_t)((nthreads + 255) / 256)), dim3(256), 0, st, a, max_run);
-    NC_HIP(hipGetLastError());
-  }
+  a.gfire = sim->d_gfire.p;
+  // nc_sim_add sizes the drive tables and the group-train ring; nothing here allocates
+  if (a.n_groups && (sim->d_groups.n < a.n_groups || sim->d_gfire.n < (size_t)NC_GRING * a.gwords))
+    return fail(NC_ERR_STATE, "drive tables were not placed");
   const uint32_t wpb = sim->cfg.waves_per_block;
   const uint32_t grid = (a.n_active_slices + wpb - 1) / wpb;
   // A fired list holds at most fcap records (one per k_neuron wave of its
@@ -1184,9 +1215,11 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
     return NC_OK;
   };
   // Whole chunks as captured graphs when nothing per-step comes from the host.
-  uint32_t done = 0;
-  const uint32_t G = sim->cfg.graph_steps;
-  if (G && ev_pos.empty() && sim->kernel_timing == 0 && max_run >= G) {
+  const uint32_t G =
+      (sim->cfg.graph_steps && ev_pos.empty() && sim->kernel_timing == 0 && max_run >= sim->cfg.graph_steps)
+          ? sim->cfg.graph_steps
+          : 0u;
+  if (G) {
     a.chunk_i = 0;
     const uint32_t key[5] = {G, wpb, grid, lds, fan_grid};
     if (!sim->graph || memcmp(&sim->graph_args, &a, sizeof a) != 0 || memcmp(sim->graph_key, key, sizeof key) != 0) {
@@ -1218,24 +1251,42 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
       sim->graph_args = a;
       memcpy(sim->graph_key, key, sizeof key);
     }
-    for (; max_run - done >= G; done += G) NC_HIP(hipGraphLaunch(sim->graph, st));
   }
-  for (uint32_t i = done; i < max_run; ++i) {
-    a.chunk_i = (int32_t)(i - done);  // chunk_off holds `done` on the device
-    a.n_ev = ev_ptr[i + 1] - ev_ptr[i];
-    a.ev_pos = sim->d_ev_pos.p + ev_ptr[i];
-    a.ev_kind = sim->d_ev_kind.p + ev_ptr[i];
-    a.use_forced = ev_pos.empty() ? 0 : 1;
-    nc_status s1 = NC_OK;
-    if (a.n_ev) {
-      void* args[] = {&a};
-      hipError_t e = hipLaunchKernel((const void*)k_drive, dim3((a.n_ev + 255) / 256), dim3(256), args, 0, st);
-      if (e != hipSuccess) s1 = fail(NC_ERR_HIP, hipGetErrorString(e));
+  // The call in sub-chunks of at most NC_GRING steps (whole graph chunks):
+  // the group spike trains of a sub-chunk are drawn into the ring first.
+  // The device chunk offset holds the steps launched before the running
+  // graph chunk or run of direct launches.
+  const uint32_t L = G ? (NC_GRING / G) * G : NC_GRING;
+  for (uint32_t c0 = 0; c0 < max_run; c0 += L) {
+    const uint32_t len = std::min(L, max_run - c0);
+    if (a.n_groups) {
+      hipLaunchKernelGGL(k_groups, dim3((a.gwords * 32 + 255) / 256, len), dim3(256), 0, st, a, c0);
+      NC_HIP(hipGetLastError());
     }
-    if (s1 == NC_OK) s1 = launch(1, fn_neuron, grid, 64 * wpb, lds);
-    if (s1 == NC_OK) s1 = launch(3, fn_fanout, fan_grid, 64 * wpb, 0);
-    if (s1 != NC_OK) return s1;
-    sim->timing_step++;
+    uint32_t j = 0;
+    if (G)
+      for (; len - j >= G; j += G) NC_HIP(hipGraphLaunch(sim->graph, st));
+    for (uint32_t i = c0 + j; i < c0 + len; ++i) {
+      a.chunk_i = (int32_t)(i - (c0 + j));
+      a.n_ev = ev_ptr[i + 1] - ev_ptr[i];
+      a.ev_pos = sim->d_ev_pos.p + ev_ptr[i];
+      a.ev_kind = sim->d_ev_kind.p + ev_ptr[i];
+      a.use_forced = ev_pos.empty() ? 0 : 1;
+      nc_status s1 = NC_OK;
+      if (a.n_ev) {
+        void* args[] = {&a};
+        hipError_t e = hipLaunchKernel((const void*)k_drive, dim3((a.n_ev + 255) / 256), dim3(256), args, 0, st);
+        if (e != hipSuccess) s1 = fail(NC_ERR_HIP, hipGetErrorString(e));
+      }
+      if (s1 == NC_OK) s1 = launch(1, fn_neuron, grid, 64 * wpb, lds);
+      if (s1 == NC_OK) s1 = launch(3, fn_fanout, fan_grid, 64 * wpb, 0);
+      if (s1 != NC_OK) return s1;
+      sim->timing_step++;
+    }
+    if (j < len && c0 + len < max_run) {
+      hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, sim->d_chunk_off, len - j);
+      NC_HIP(hipGetLastError());
+    }
   }
   sim->gstep += max_run;
   if (!timed_kinds.empty()) {
@@ -1309,8 +1360,14 @@ extern "C" nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_even
     if (kind[k] > 1) return fail(NC_ERR_ARG, "kind must be 0 (input) or 1 (background)");
   }
   for (uint32_t k = 0; k < n_events; ++k)
+    if (step_off[k] >= (uint32_t)NC_MAX_STEPS - r->steps_done) return fail(NC_ERR_RANGE, "event step out of range");
+  const size_t before = r->events.size();
+  for (uint32_t k = 0; k < n_events; ++k)
     r->events.push_back({r->steps_done + step_off[k], {r->pos(ids[k]), kind[k]}});
-  return NC_OK;
+  NC_HIP(hipSetDevice(sim->device));
+  st = reserve_events(sim);
+  if (st != NC_OK) r->events.resize(before);  // all or nothing
+  return st;
 }
 NC_ABI_CATCH
 
@@ -1594,9 +1651,14 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   std::vector<int32_t> lf((size_t)npos, NC_NEVER);
   std::vector<uint32_t> fi((size_t)npos, 0u), asum((size_t)npos, 0u);
   NC_HIP(hipMemcpy(V.data(), sim->V + pos0, npos * 4, hipMemcpyDeviceToHost));
+  // Step values a checkpoint may hold: NC_NEVER, or a step of the run so far
+  // (arrivals: up to the longest delay ahead); anything else would index the
+  // step tables out of range.
+  auto past = [s](int32_t v) { return v == NC_NEVER || (v >= 1 && v <= s); };
   for (uint32_t i = 0; i < h.n; ++i) {
     const uint32_t pl = r->maps->pos_of_id[i];
     memcpy(&V[pl], p, 4), memcpy(&lf[pl], p + 4, 4), memcpy(&fi[pl], p + 8, 4);
+    if (!past(lf[pl])) return fail(NC_ERR_ARG, "checkpoint last fire out of range");
     p += 12;
   }
   // synapses, and from their arrivals the active masks, summaries, pending
@@ -1616,6 +1678,8 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
       nc_slot& x = hs[r->maps->slot_of_syn[syn]];
       int32_t arr;
       memcpy(&x.w, e, 4), memcpy(&x.depol, e + 4, 4), memcpy(&x.last_arr, e + 8, 4), memcpy(&arr, e + 12, 4);
+      if (!past(x.last_arr)) return fail(NC_ERR_ARG, "checkpoint last arrival out of range");
+      if (arr != NC_NEVER && (arr < 1 || arr > s + 255)) return fail(NC_ERR_ARG, "checkpoint arrival out of range");
       x.arr_inh = arr * 2 + (x.arr_inh & 1);
       edge_arr[syn] = arr;
       if (arr == NC_NEVER) continue;
@@ -1634,8 +1698,15 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
     uint32_t off, id, kind;
     memcpy(&off, p, 4), memcpy(&id, p + 4, 4), memcpy(&kind, p + 8, 4);
     p += 12;
-    if (id >= r->n || off < 1) return fail(NC_ERR_ARG, "checkpoint event out of range");
+    if (id >= r->n || off < 1 || kind > 1 || off >= (uint32_t)NC_MAX_STEPS - h.steps_done)
+      return fail(NC_ERR_ARG, "checkpoint event out of range");
     events.push_back({h.steps_done + off, {r->pos(id), (uint8_t)kind}});
+  }
+  r->events = events;
+  st = reserve_events(sim);
+  if (st != NC_OK) {
+    r->events.clear();
+    return st;
   }
   // upload
   hipStream_t sq = sim->stream;
@@ -1669,7 +1740,6 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipStreamSynchronize(sq));
   r->steps_done = h.steps_done;
   r->lr = h.lr, r->pre_f = h.pre_f, r->post_f = h.post_f;
-  r->events = events;
   return NC_OK;
 }
 NC_ABI_CATCH
@@ -1706,15 +1776,91 @@ extern "C" nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n) tr
 }
 NC_ABI_CATCH
 
+namespace {
+// The simulation's state at a step boundary, for the autotuner: a device copy
+// of every buffer the step kernels write, and the host-side step counts and
+// queued events.  Restoring it puts the simulation back exactly where it was.
+struct StateCopy {
+  struct Buf {
+    void* dev;
+    void* copy;
+    size_t bytes;
+  };
+  std::vector<Buf> bufs;
+  struct NetState {
+    uint32_t steps_done, steps_left;
+    std::vector<std::pair<uint32_t, std::pair<uint32_t, uint8_t>>> events;
+  };
+  std::vector<NetState> nets;
+  uint64_t gstep = 0, timing_step = 0, hn = 0, hs = 0, hm = 0;
+  ~StateCopy() {
+    for (Buf& b : bufs)
+      if (b.copy) (void)hipFree(b.copy);
+  }
+};
+
+nc_status state_save(nc_sim* sim, StateCopy* c) {
+  const uint64_t P = sim->max_pos, Q = P / 64;
+  const std::pair<void*, size_t> regions[] = {
+      {sim->slots, sim->max_slots * sizeof(nc_slot)},
+      {sim->amask, sim->max_mwords * 4},
+      {sim->asum, P * 4},
+      {sim->V, P * 4},
+      {sim->last_fire, P * 4},
+      {sim->firings, P * 4},
+      {sim->edges, sim->max_out * sizeof(nc_edge)},
+      {sim->pend0, sim->max_slots * 4},
+      {sim->pend1, sim->max_slots * 4},
+      {sim->pend_cnt, Q * sizeof(uint2)},
+      {sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)},
+      {sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4},
+      {sim->forced_in, (P / 32) * 4},
+      {sim->forced_bg, (P / 32) * 4},
+      {sim->raster_cnt, 64},
+      {sim->ctr, sim->ctr_rows * NC_NCTR * 8},
+  };
+  for (const auto& r : regions) {
+    StateCopy::Buf b{r.first, nullptr, r.second};
+    NC_HIP(hipMalloc(&b.copy, b.bytes));
+    c->bufs.push_back(b);
+    NC_HIP(hipMemcpyAsync(b.copy, b.dev, b.bytes, hipMemcpyDeviceToDevice, sim->stream));
+  }
+  for (const NetRec& r : sim->nets) c->nets.push_back({r.steps_done, r.steps_left, r.events});
+  c->gstep = sim->gstep, c->timing_step = sim->timing_step;
+  c->hn = sim->host_neuron_steps, c->hs = sim->host_synapse_steps, c->hm = sim->host_mword_steps;
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return NC_OK;
+}
+
+nc_status state_restore(nc_sim* sim, const StateCopy& c) {
+  for (const StateCopy::Buf& b : c.bufs)
+    NC_HIP(hipMemcpyAsync(b.dev, b.copy, b.bytes, hipMemcpyDeviceToDevice, sim->stream));
+  for (size_t i = 0; i < c.nets.size() && i < sim->nets.size(); ++i) {
+    sim->nets[i].steps_done = c.nets[i].steps_done;
+    sim->nets[i].steps_left = c.nets[i].steps_left;
+    sim->nets[i].events = c.nets[i].events;
+  }
+  sim->gstep = c.gstep, sim->timing_step = c.timing_step;
+  sim->host_neuron_steps = c.hn, sim->host_synapse_steps = c.hs, sim->host_mword_steps = c.hm;
+  return NC_OK;
+}
+
+nc_status check_launch(const nc_launch_cfg& c) {
+  const uint32_t w = c.waves_per_block;
+  if (w != 1 && w != 2 && w != 4 && w != 8) return fail(NC_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
+  if (c.xcd_swizzle > 1) return fail(NC_ERR_ARG, "xcd_swizzle must be 0 or 1");
+  if (c.graph_steps > NC_GRING) return fail(NC_ERR_ARG, "graph_steps must be at most 1024");
+  if (c.stage_tables > 1) return fail(NC_ERR_ARG, "stage_tables must be 0 or 1");
+  if (c.fanout_waves && (c.fanout_waves < 64 || c.fanout_waves > (1u << 20)))
+    return fail(NC_ERR_ARG, "fanout_waves must be 0 or in [64, 2^20]");
+  return NC_OK;
+}
+}  // namespace
+
 extern "C" nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg) try {
   if (!sim || !cfg) return fail(NC_ERR_ARG, "null argument");
-  const uint32_t w = cfg->waves_per_block;
-  if (w != 1 && w != 2 && w != 4 && w != 8) return fail(NC_ERR_ARG, "waves_per_block must be 1, 2, 4 or 8");
-  if (cfg->xcd_swizzle > 1) return fail(NC_ERR_ARG, "xcd_swizzle must be 0 or 1");
-  if (cfg->graph_steps > 4096) return fail(NC_ERR_ARG, "graph_steps must be at most 4096");
-  if (cfg->stage_tables > 1) return fail(NC_ERR_ARG, "stage_tables must be 0 or 1");
-  if (cfg->fanout_waves && (cfg->fanout_waves < 64 || cfg->fanout_waves > (1u << 20)))
-    return fail(NC_ERR_ARG, "fanout_waves must be 0 or in [64, 2^20]");
+  nc_status st = check_launch(*cfg);
+  if (st != NC_OK) return st;
   sim->cfg = *cfg;
   return NC_OK;
 }
@@ -1727,9 +1873,24 @@ extern "C" nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg) tr
 }
 NC_ABI_CATCH
 
-extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen) try {
-  if (!sim || steps_per_trial == 0) return fail(NC_ERR_ARG, "null sim or zero steps");
+extern "C" nc_status nc_sim_tune_list(nc_sim* sim, const nc_launch_cfg* cands, uint32_t n_cands,
+                                      uint32_t steps_per_trial, uint32_t rounds, nc_launch_cfg* chosen,
+                                      float* ms_out) try {
+  if (!sim || !cands || n_cands == 0 || steps_per_trial == 0 || rounds == 0)
+    return fail(NC_ERR_ARG, "null sim, no candidates, zero steps or zero rounds");
+  for (uint32_t i = 0; i < n_cands; ++i) {
+    nc_status cs = check_launch(cands[i]);
+    if (cs != NC_OK) return cs;
+  }
   NC_HIP(hipSetDevice(sim->device));
+  if (sim->upload_pending) {
+    NC_HIP(hipEventSynchronize(sim->upload_done));
+    sim->upload_pending = false;
+  }
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  StateCopy saved;
+  nc_status st = state_save(sim, &saved);
+  if (st != NC_OK) return st;
   hipEvent_t e0 = nullptr, e1 = nullptr;
   NC_HIP(hipEventCreate(&e0));
   if (hipEventCreate(&e1) != hipSuccess) {
@@ -1737,39 +1898,71 @@ extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launc
     return fail(NC_ERR_HIP, "hipEventCreate failed");
   }
   const nc_launch_cfg start = sim->cfg;
-  nc_launch_cfg best = start;
-  float best_ms = -1.0f;
-  nc_status st = NC_OK;
-  const uint32_t waves[4] = {1, 2, 4, 8};
-  const uint32_t graphs[2] = {0, steps_per_trial};
-  for (uint32_t g : graphs)
-    for (uint32_t w : waves)
-      for (uint32_t x = 0; x < 4 && st == NC_OK; ++x) {
-        sim->cfg = nc_launch_cfg{w, x & 1u, g, x >> 1, start.fanout_waves};
-        st = nc_sim_step(sim, steps_per_trial);  // warm-up (and graph capture)
-        if (st != NC_OK) break;
-        if (hipEventRecord(e0, sim->stream) != hipSuccess) {
-          st = fail(NC_ERR_HIP, "hipEventRecord failed");
-          break;
-        }
-        st = nc_sim_step(sim, steps_per_trial);
-        if (st != NC_OK) break;
-        float ms = 0.0f;
-        if (hipEventRecord(e1, sim->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-          st = fail(NC_ERR_HIP, "event timing failed");
-          break;
-        }
-        if (best_ms < 0.0f || ms < best_ms) {
-          best_ms = ms;
-          best = sim->cfg;
-        }
+  const int timing = sim->kernel_timing;
+  sim->kernel_timing = 0;  // events on the launches would keep graphs out
+  std::vector<std::vector<float>> ms((size_t)n_cands);
+  // Every trial starts from the saved state: a warm-up trial (which also
+  // captures the candidate's graph), the state put back, then the timed
+  // trial.  Rounds visit the candidates in rotated order.
+  for (uint32_t rd = 0; rd < rounds && st == NC_OK; ++rd)
+    for (uint32_t k = 0; k < n_cands && st == NC_OK; ++k) {
+      const uint32_t i = (k + rd * (n_cands / rounds + 1)) % n_cands;
+      sim->cfg = cands[i];
+      st = state_restore(sim, saved);
+      if (st == NC_OK) st = nc_sim_step(sim, steps_per_trial);
+      if (st == NC_OK) st = state_restore(sim, saved);
+      if (st != NC_OK) break;
+      if (hipEventRecord(e0, sim->stream) != hipSuccess) {
+        st = fail(NC_ERR_HIP, "hipEventRecord failed");
+        break;
       }
+      st = nc_sim_step(sim, steps_per_trial);
+      if (st != NC_OK) break;
+      float t = 0.0f;
+      if (hipEventRecord(e1, sim->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+          hipEventElapsedTime(&t, e0, e1) != hipSuccess) {
+        st = fail(NC_ERR_HIP, "event timing failed");
+        break;
+      }
+      ms[i].push_back(t);
+    }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  sim->cfg = st == NC_OK ? best : start;
+  sim->kernel_timing = timing;
+  nc_status rs = state_restore(sim, saved);
+  if (rs == NC_OK && hipStreamSynchronize(sim->stream) != hipSuccess) rs = fail(NC_ERR_HIP, "synchronize failed");
+  if (st == NC_OK) st = rs;
+  if (st != NC_OK) {
+    sim->cfg = start;
+    return st;
+  }
+  uint32_t best = 0;
+  float best_ms = -1.0f;
+  for (uint32_t i = 0; i < n_cands; ++i) {
+    std::vector<float> v = ms[i];
+    std::sort(v.begin(), v.end());
+    const float med = v.empty() ? 0.0f : v[v.size() / 2];
+    if (ms_out) ms_out[i] = med;
+    if (best_ms < 0.0f || med < best_ms) {
+      best_ms = med;
+      best = i;
+    }
+  }
+  sim->cfg = cands[best];
   if (chosen) *chosen = sim->cfg;
-  return st;
+  return NC_OK;
+}
+NC_ABI_CATCH
+
+extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen) try {
+  if (!sim || steps_per_trial == 0) return fail(NC_ERR_ARG, "null sim or zero steps");
+  std::vector<nc_launch_cfg> cands;
+  const uint32_t waves[4] = {1, 2, 4, 8};
+  const uint32_t graphs[2] = {0, std::min<uint32_t>(steps_per_trial, NC_GRING)};
+  for (uint32_t g : graphs)
+    for (uint32_t w : waves)
+      for (uint32_t x = 0; x < 4; ++x) cands.push_back(nc_launch_cfg{w, x & 1u, g, x >> 1, sim->cfg.fanout_waves});
+  return nc_sim_tune_list(sim, cands.data(), (uint32_t)cands.size(), steps_per_trial, 3, chosen, nullptr);
 }
 NC_ABI_CATCH
 

@@ -46,6 +46,9 @@
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
 #define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
+// Group spike trains are drawn for at most NC_GRING steps ahead, into a ring
+// of that many rows (a call of more steps draws them chunk by chunk).
+#define NC_GRING 1024u
 
 // Per-network descriptor, rewritten by the host before each batch of steps.
 struct nc_netdesc {
@@ -100,7 +103,7 @@ struct nc_step_args {
   const nc_netdesc* nets;
   const nc_group* groups;
   const uint32_t* thr;
-  uint32_t* gfire;  // [chunk step][group words]: group spike trains of this batch
+  uint32_t* gfire;  // [step of the call & (NC_GRING - 1)][group words]: group spike trains
   uint32_t gwords;  // words per step in gfire
   uint32_t n_groups;
   uint32_t* forced_in;
@@ -249,12 +252,14 @@ NC_HD uint32_t nc_slot_of(uint32_t slot_off, uint32_t l, uint32_t k) {
   return slot_off + (k >> NC_CHUNK_LOG) * (64u * NC_CHUNK) + l * NC_CHUNK + (k & (NC_CHUNK - 1u));
 }
 
-// Group spike trains of a batch of steps: one Philox draw per (step, group),
-// shared by every member of the group.
-__global__ void k_groups(nc_step_args a, uint32_t n_steps) {
-  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n_steps * a.gwords * 32u) return;
-  const uint32_t i = t / (a.gwords * 32u), g = t % (a.gwords * 32u);
+// Group spike trains of steps c0 .. c0 + gridDim.y - 1 of a call (at most
+// NC_GRING steps): one Philox draw per (step, group), shared by every member
+// of the group, into ring row (step & (NC_GRING - 1)).  blockIdx.y is the
+// step, blockIdx.x * blockDim.x + threadIdx.x the group bit.
+__global__ __launch_bounds__(256) void k_groups(nc_step_args a, uint32_t c0) {
+  const uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.gwords * 32u) return;  // whole waves: gwords * 32 is a multiple of 32, blocks of 256
+  const uint32_t i = c0 + blockIdx.y;
   bool fire = false;
   if (g < a.n_groups) {
     const nc_group G = a.groups[g];
@@ -268,8 +273,9 @@ __global__ void k_groups(nc_step_args a, uint32_t n_steps) {
   }
   const unsigned long long m = __ballot(fire);
   const uint32_t lane = nc_lane();
-  if (lane == 0) a.gfire[(size_t)i * a.gwords + (g >> 5)] = (uint32_t)m;
-  if (lane == 32) a.gfire[(size_t)i * a.gwords + (g >> 5)] = (uint32_t)(m >> 32);
+  uint32_t* const row = a.gfire + (size_t)(i & (NC_GRING - 1u)) * a.gwords;
+  if (lane == 0) row[g >> 5] = (uint32_t)m;
+  if (lane == 32) row[g >> 5] = (uint32_t)(m >> 32);
 }
 
 // Host-scheduled forced fires of this step -> bitmasks read by k_neuron.
@@ -403,7 +409,7 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   bool f_in = false, f_bg = false;
   if (valid && (dw & NC_DEG_INPUT)) {
     const uint32_t g = nw.y;
-    f_in = ((a.gfire[(size_t)ci * a.gwords + (g >> 5)] >> (g & 31)) & 1u) != 0;
+    f_in = ((a.gfire[(size_t)((uint32_t)ci & (NC_GRING - 1u)) * a.gwords + (g >> 5)] >> (g & 31)) & 1u) != 0;
   }
   if (valid && nd.bg_thr) {
     const uint64_t seed = ((uint64_t)nd.bg_hi << 32) | nd.bg_lo;

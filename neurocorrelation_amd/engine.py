@@ -227,11 +227,25 @@ class Simulation:
         return _cfg_dict(cfg)
 
     def tune(self, steps_per_trial=32):
-        """Autotunes the launch shape on the live simulation (advances it by
-        2 * 32 * steps_per_trial steps); returns the chosen shape."""
+        """Autotunes the launch shape over the engine's 32 candidate shapes,
+        every trial from the same saved state; the simulation is left where
+        it was.  Returns the chosen shape."""
         cfg = LaunchCfg()
         check(lib().nc_sim_tune(self._h, steps_per_trial, C.byref(cfg)))
         return _cfg_dict(cfg)
+
+    def tune_list(self, candidates, steps_per_trial=16, rounds=3):
+        """Autotunes over the given shapes (dicts of LaunchCfg fields); returns
+        (chosen shape, median ms per candidate)."""
+        arr = (LaunchCfg * len(candidates))()
+        for i, c in enumerate(candidates):
+            arr[i] = LaunchCfg(c.get("waves_per_block", 4), c.get("xcd_swizzle", 0), c.get("graph_steps", 0),
+                               c.get("stage_tables", 0), c.get("fanout_waves", 0))
+        ms = np.zeros(len(candidates), np.float32)
+        cfg = LaunchCfg()
+        check(lib().nc_sim_tune_list(self._h, arr, len(candidates), steps_per_trial, rounds, C.byref(cfg),
+                                     ptr(ms, C.c_float)))
+        return _cfg_dict(cfg), ms
 
     def save(self, net_id):
         """The network's dynamic state as bytes (a checkpoint)."""

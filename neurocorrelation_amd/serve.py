@@ -196,7 +196,10 @@ class Server:
                     try:
                         job.nid = ln.sim.add(job.net, job.params, job.drive, steps=job.steps)
                     except NCError as e:
-                        if e.code != 5:  # anything but NC_ERR_CAPACITY
+                        # no room in the lane's pools (NC_ERR_CAPACITY) or for
+                        # its staging upload (NC_ERR_OOM): wait for room;
+                        # anything else is an error
+                        if e.code not in (4, 5):
                             raise
                         if ln.running:  # wait for room
                             break

@@ -75,12 +75,49 @@ def test_launch_shapes_give_identical_results(mix_and_ref, cfg):
 
 
 def test_tuned_launch_gives_identical_results(mix_and_ref):
+    """Tuning takes no simulated step: the tuned run's 1000 steps equal the
+    default launch's."""
     nets, budgets, ref = mix_and_ref
     say("autotune, 8 steps per trial")
-    got, chosen = _run(nets, budgets, tune=8, calls=(488,))  # 64 trials x 8 steps + 488 = 1000
+    got, chosen = _run(nets, budgets, tune=8)
     say("chosen", chosen)
     assert chosen["waves_per_block"] in (1, 2, 4, 8)
     _same(got, ref)
+
+
+def test_tuner_times_every_candidate_from_the_same_state():
+    """A deliberately slow first candidate (fan-out capped at 64 waves on a
+    network firing at p ~ 0.27 per step, so each fan-out wave takes five
+    slices' fired lanes in turn) loses to the default shape; the tuner leaves
+    the simulation exactly where it was, and the run continues as an untuned
+    one does."""
+    net, drive, params, _ = W.build("plain-20k", seed=3)
+
+    def fresh():
+        sim = nc.Simulation.for_network(net, raster_capacity=1 << 24)
+        i = sim.add(net, params, drive)
+        sim.step(300)  # into the busy regime
+        return sim, i
+
+    sim, i = fresh()
+    before = (sim.raster(i, 1 << 24), sim.potential(i, net.n), sim.weights(i, net.n_syn), sim.progress(i))
+    cands = [dict(waves_per_block=1, xcd_swizzle=1, fanout_waves=64), dict(waves_per_block=4),
+             dict(waves_per_block=2), dict(waves_per_block=4, graph_steps=16)]
+    chosen, ms = sim.tune_list(cands, steps_per_trial=16, rounds=3)
+    say("tuner medians (ms per 16 steps)", ms.tolist(), "chosen", chosen)
+    assert chosen["fanout_waves"] != 64
+    assert ms[0] > 1.2 * ms[1:].min()
+    after = (sim.raster(i, 1 << 24), sim.potential(i, net.n), sim.weights(i, net.n_syn), sim.progress(i))
+    assert before[3] == after[3] == (300, 0xFFFFFFFF)
+    assert len(after[0][1]) == 0  # no step was simulated (the raster was drained before)
+    assert np.array_equal(before[1].view(np.uint32), after[1].view(np.uint32))
+    assert np.array_equal(before[2].view(np.uint32), after[2].view(np.uint32))
+    sim.step(200)
+    ref, j = fresh()
+    ref.raster(j, 1 << 24)
+    ref.step(200)
+    _same([(sim.raster(i, 1 << 24), sim.potential(i, net.n), sim.weights(i, net.n_syn), sim.progress(i))],
+          [(ref.raster(j, 1 << 24), ref.potential(j, net.n), ref.weights(j, net.n_syn), ref.progress(j))])
 
 
 def test_graph_chunks_with_host_events():
