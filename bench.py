@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "synapse updates/s (% HBM roofline); served sims/s + p50/p99 vs plain"
 HBM_PEAK_GBS = 8000.0
-TIMING_EVERY = 10
+TIMING_EVERY = 10  # kernel events on every 10th step's launches from 200 timed steps on, every step below
 
 
 def neuron_kernel_bytes(n_neurons, c0, c1, steps):
@@ -50,6 +50,22 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
          + 4.0 * d["pending_scanned"])
     return b, {"A": d["active_slot_steps"], "R": d["arrivals"], "F": d["fired"], "F_in": d["fired_in_degree"],
                "pending": d["pending_scanned"], "mask_words": d["mask_words_read"]}
+
+
+def fanout_kernel_bytes(c0, c1, steps):
+    """Algorithmic bytes per launch of the fan-out kernel (k_fanout), from the
+    engine's event counters over the timed steps:
+      16 B per fired-slice record   {mask lo, mask hi, step, slice} read
+       8 B per fired neuron         out-row head {first edge, out-degree}
+      16 B per out-edge of a        edge record {target slot, target slice |
+           fired neuron             delay, latched arrival, target slice base}
+      32 B per accepted spike       target slot w + arrival word read 8,
+                                    depol + arrival write 8, edge latch write 4,
+                                    pending entry write 4, pending count RMW 8"""
+    d = {k: (c1[k] - c0[k]) / steps for k in ("fired_records", "fired", "fired_out_degree", "accepted_spikes")}
+    b = 16.0 * d["fired_records"] + 8.0 * d["fired"] + 16.0 * d["fired_out_degree"] + 32.0 * d["accepted_spikes"]
+    return b, {"records": d["fired_records"], "F": d["fired"], "F_out": d["fired_out_degree"],
+               "accepted": d["accepted_spikes"]}
 
 
 def copy_calibration(mib):
@@ -89,11 +105,15 @@ def drive_fires(drive, ids, groups, n_corr_groups, steps, keep):
     return [sorted(x) for x in per]
 
 
-def cpu_baseline(net, drive, budget_s=12.0, sample_neurons=40_000):
+def cpu_baseline(net, drive, budget_s=12.0, sample_neurons=10_000, timed_steps=30):
     """The reference's NeuCor::run() (oracle/_ref/ref_harness, built from the
     reference's src/NeuCor.cpp) on the first `sample_neurons` neurons of the
     workload (a compact region, ids follow a Morton curve) with their
-    internal synapses; single thread."""
+    internal synapses, `timed_steps` steps after 100 warm-up steps; single
+    thread.  The whole 10^6-neuron network is not used: the reference builds
+    it in O(S K) createSynapse calls and steps it in seconds per step, which
+    would keep the benchmark busy for many minutes; its cost per synapse-step
+    is that of any region of the same density and degree."""
     from oracle import oracle as O
     from oracle import ref as R
     ex = net.export()
@@ -118,16 +138,29 @@ def cpu_baseline(net, drive, budget_s=12.0, sample_neurons=40_000):
     except OSError:
         pass
     if R.available():
-        probe = R.run_bench(sub, 3, inputs=drive_fires(drive, ids, groups, ncorr, 3, set(range(M))))
-        per_step = max(probe["run_s"] / 3, 1e-6)
-        steps = int(max(5, min(2000, budget_s / per_step)))
-        res = R.run_bench(sub, steps, inputs=drive_fires(drive, ids, groups, ncorr, steps, set(range(M))))
+        warm = 100
+
+        def drive_for(total):
+            # the workload's input fires and raw background draws (the
+            # reference applies its own refractory check to the latter)
+            inputs = drive_fires(drive, ids, groups, ncorr, total, set(range(M)))
+            bg = [[] for _ in range(total)]
+            if drive.bg_prob > 0:
+                es, ei = O.bg_events(drive.seed, drive.bg_prob, M, 0, total)
+                for s_, i_ in zip(es.tolist(), ei.tolist()):
+                    bg[s_ - 1].append(i_)
+            return inputs, bg
+
+        steps = timed_steps
+        inp, bg = drive_for(warm + steps)
+        res = R.run_bench(sub, steps, inputs=inp, bg=bg, warmup=warm)
         return {"value": S * steps / res["run_s"], "unit": "synapse-updates/s", "cores": cores,
                 "kind": "reference",
-                "sample": "reference NeuCor::run() (runAll, runSpeed = 1 ms, STDP on, sign flags repaired) "
-                          "on the first %d neurons of the workload network and their %d internal synapses, "
-                          "%d steps after construction; 1 thread on %s (nproc %d)"
-                          % (M, S, steps, cpu_model, os.cpu_count() or 0)}
+                "sample": "reference NeuCor::run() (runAll, runSpeed = 1 ms, STDP on, sign flags repaired, the "
+                          "workload's input and background fires) on the first %d neurons of the workload network "
+                          "(a compact Morton region) and their %d internal synapses, %d timed steps after %d "
+                          "warm-up steps; 1 thread on %s (nproc %d)"
+                          % (M, S, steps, warm, cpu_model, os.cpu_count() or 0)}
     o = O.Sim(sub)
     o.set_drive(ids[ids < M], groups[ids < M], ncorr, drive.period, drive.p0, drive.amp_corr, drive.amp_ind,
                 0.0, drive.seed)
@@ -183,11 +216,13 @@ def bench_batched(args):
     sim.synchronize()
     tune_table, tune_s = None, 0.0
     if args.tune_steps:
-        # the engine's 32 candidate shapes, each timed from the same saved
+        # the engine's 38 candidate shapes, each timed from the same saved
         # state over 3 rounds (tuning takes no simulated step)
         t_tune = time.perf_counter()
+        gs = (0, min(args.tune_steps, 1024))
         cands = [dict(waves_per_block=w, xcd_swizzle=x & 1, graph_steps=g, stage_tables=x >> 1)
-                 for g in (0, min(args.tune_steps, 1024)) for w in (1, 2, 4, 8) for x in range(4)]
+                 for g in gs for w in (1, 2, 4, 8) for x in range(4)]
+        cands += [dict(waves_per_block=w, graph_steps=g, persistent=1) for g in gs for w in (2, 4, 8)]
         chosen, ms = sim.tune_list(cands, steps_per_trial=args.tune_steps, rounds=3)
         tune_s = time.perf_counter() - t_tune
         tune_table = [dict(c, median_ms=float(m)) for c, m in zip(cands, ms)]
@@ -342,6 +377,8 @@ def main():
     ap.add_argument("--tune-steps", type=int, default=16,
                     help="batched-*: steps per autotuner trial (0: no autotuning, --waves-per-block launch)")
     ap.add_argument("--waves-per-block", type=int, default=4, help="batched-*: launch shape when not autotuning")
+    ap.add_argument("--launch", default="",
+                    help='plain workloads: launch shape as JSON, e.g. {"persistent": 1, "waves_per_block": 8}')
     ap.add_argument("--rate", type=float, default=200.0, help="served: job arrivals per second")
     ap.add_argument("--window", type=float, default=60.0, help="served: seconds of arrivals")
     ap.add_argument("--max-neurons", type=int, default=100_000, help="served: largest job network")
@@ -384,9 +421,11 @@ def main():
     N, S = info["n"], info["n_syn"]
     sim = nc.Simulation.for_network(net)
     nid = sim.add(net, params, drive)
+    launch = json.loads(args.launch) if args.launch else {}
+    sim.set_launch(**launch)
     setup_s = time.perf_counter() - t0
-    print("[bench] %s: %d neurons, %d synapses, setup %.1fs" % (args.workload, N, S, setup_s),
-          file=sys.stderr, flush=True)
+    print("[bench] %s: %d neurons, %d synapses, setup %.1fs, launch %s" % (args.workload, N, S, setup_s,
+          sim.launch()), file=sys.stderr, flush=True)
 
     def run(sim_, n):
         done_ = 0
@@ -397,15 +436,18 @@ def main():
 
     run(sim, args.warmup)
     sim.synchronize()
+    every = TIMING_EVERY if args.steps >= 200 else 1
     if not args.no_kernel_timing:
-        # start/stop events carried by the neuron-step dispatch of every
-        # TIMING_EVERY-th step (events on every launch would slow the queue)
-        sim.set_kernel_timing(1 << 1, TIMING_EVERY)
+        # start/stop events carried by the neuron-step and fan-out dispatches
+        # (of every TIMING_EVERY-th step on long runs: events on every launch
+        # slow the queue a little)
+        sim.set_kernel_timing((1 << 1) | (1 << 3), every)
     t_start = time.perf_counter()
     run(sim, args.steps)
     sim.synchronize()
     elapsed = time.perf_counter() - t_start
-    launches, k_ms = sim.kernel_timing(1)
+    timing = {k: sim.kernel_timing(k) for k in (1, 3)}
+    launch_used = sim.launch()
     sim.set_kernel_timing(0)
     print("[bench] timed %d steps in %.3fs" % (args.steps, elapsed), file=sys.stderr, flush=True)
     if not args.no_kernel_timing:
@@ -419,31 +461,56 @@ def main():
 
     # Event counts of the timed steps: the same network, drive and steps again
     # with the device counters on (the step is deterministic; counting is kept
-    # out of the timed run).
+    # out of the timed run); firing counts split inputs from the rest.
+    split = {"input_fired_per_step": None, "network_fired_per_step": None}
     if args.no_counts:
         c0 = c1 = None
     else:
         sim = nc.Simulation.for_network(net)
-        sim.add(net, params, drive)
+        cid = sim.add(net, params, drive)
         run(sim, args.warmup)
         sim.set_counting(True)
         c0 = sim.counters()
+        f0 = sim.firings(cid, N).astype(np.int64)
         run(sim, args.steps)
         sim.synchronize()
         c1 = sim.counters()
+        df = sim.firings(cid, N).astype(np.int64) - f0
         sim.close()
+        sel, _ = net.select_inputs(drive)
+        is_in = np.zeros(N, bool)
+        is_in[sel] = True
+        split = {"input_fired_per_step": float(df[is_in].sum()) / args.steps,
+                 "network_fired_per_step": float(df[~is_in].sum()) / args.steps}
 
     value = S * args.steps / elapsed
-    if c0 is not None:
-        bytes_launch, ev = neuron_kernel_bytes(N, c0, c1, args.steps)
-    else:
-        bytes_launch, ev = None, {k: None for k in ("A", "R", "F", "F_in", "pending", "mask_words")}
-    avg_launch_s = (k_ms / 1000.0) / launches if launches else None
-    achieved = bytes_launch / avg_launch_s / 1e9 if (avg_launch_s and bytes_launch) else None
+    kernels = {}
+
+    def step_bytes():
+        # the persistent form's one launch per step: both phases
+        b1, u1 = neuron_kernel_bytes(N, c0, c1, args.steps)
+        b2, u2 = fanout_kernel_bytes(c0, c1, args.steps)
+        return b1 + b2, dict(u1, **{"records": u2["records"], "F_out": u2["F_out"], "accepted": u2["accepted"]})
+
+    models = (((1, "k_step", step_bytes),) if launch_used.get("persistent") else
+              ((1, "k_neuron", lambda: neuron_kernel_bytes(N, c0, c1, args.steps)),
+               (3, "k_fanout", lambda: fanout_kernel_bytes(c0, c1, args.steps))))
+    for kind, name, model in models:
+        launches, k_ms = timing[kind]
+        b, units = model() if c0 is not None else (None, None)
+        avg = (k_ms / 1000.0) / launches if launches else None
+        ach = b / avg / 1e9 if (avg and b) else None
+        kernels[name] = {"algorithmic_bytes_per_launch": b, "units_per_step": units,
+                         "avg_launch_us": avg * 1e6 if avg else None, "launches": launches,
+                         "achieved": ach, "frac": ach / HBM_PEAK_GBS if ach else None}
+    first = "k_step" if "k_step" in kernels else "k_neuron"
+    ev = kernels[first]["units_per_step"] or {k: None for k in ("A", "R", "F", "F_in", "pending")}
+    # the dominant kernel: the larger share of the step's device time
+    dom = max(kernels, key=lambda k: kernels[k]["avg_launch_us"] or 0.0)
     traffic = None
     tpath = os.path.join(ROOT, "profiles", "traffic_%s.json" % args.workload)
     if os.path.exists(tpath):
-        traffic = json.load(open(tpath)).get("neuron_step_bytes_per_launch")
+        traffic = json.load(open(tpath)).get("%s_bytes_per_launch" % dom)
     rate = ev["F"] / N if ev["F"] is not None else None
 
     cpu = None
@@ -452,6 +519,7 @@ def main():
             cpu = cpu_baseline(net, drive)
         except Exception as e:  # reported, never fatal for the GPU number
             print("[bench] cpu baseline failed: %r" % (e,), file=sys.stderr)
+    K = kernels[dom]
     out = {
         "metric": METRIC,
         "value": value,
@@ -465,14 +533,17 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": args.workload, "neurons": N, "synapses": S, "dt_ms": params.dt,
-                   "stdp": True, "seed": 0, "fired_per_step": ev["F"], "rate_per_step": rate,
-                   "arrivals_per_step": ev["R"], "active_slots_per_step": ev["A"],
-                   "pending_per_step": ev["pending"]},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS if achieved else None, "traffic": traffic,
-                     "kernel": "k_neuron", "algorithmic_bytes_per_launch": bytes_launch,
-                     "avg_launch_us": avg_launch_s * 1e6 if avg_launch_s else None, "launches": launches},
+        "config": dict({"workload": args.workload, "neurons": N, "synapses": S, "dt_ms": params.dt,
+                        "stdp": True, "seed": 0, "fired_per_step": ev["F"], "rate_per_step": rate,
+                        "arrivals_per_step": ev["R"], "active_slots_per_step": ev["A"],
+                        "pending_per_step": ev["pending"], "drive_p0": drive.p0, "bg_prob": drive.bg_prob,
+                        "launch": launch_used},
+                       **split),
+        "roofline": {"bound": "hbm", "achieved": K["achieved"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": K["frac"], "traffic": traffic, "kernel": dom,
+                     "algorithmic_bytes_per_launch": K["algorithmic_bytes_per_launch"],
+                     "avg_launch_us": K["avg_launch_us"], "launches": K["launches"]},
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -260,11 +260,12 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * [2] active slot-steps, [3] arrivals, [4] fired neurons,
  * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes,
  * [8] pending-list entries scanned, [9] active-mask words read,
- * [10] active-mask words written back.
+ * [10] active-mask words written back, [11] fired-slice records (64-neuron
+ * slices with at least one fired neuron, summed over steps).
  * [0], [1] and [9] are kept by the host; the others are device counters that
  * advance only while counting is on (nc_set_counting).
  * Writes the first n (at most NC_NUM_COUNTERS) into out. */
-#define NC_NUM_COUNTERS 11
+#define NC_NUM_COUNTERS 12
 nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
 
 /* Checkpoint of one network's dynamic state.  No counterpart in the
@@ -295,23 +296,32 @@ nc_status nc_set_counting(nc_sim* sim, int enable);
  * many synapses see STDP per step) instead of reading them through the
  * caches; fanout_waves: the most waves of the fan-out kernel, which share
  * the step's records of slices with fired neurons (below the cap, one
- * record per wave; never more waves than records can need).  Results do
- * not depend on the launch shape.  Default
- * {4, 0, 0, 0, 0}; nc_sim_tune keeps fanout_waves as it finds it. */
+ * record per wave; never more waves than records can need); persistent:
+ * the whole step (neuron phase, a grid-wide barrier, fan-out) in one launch
+ * of a grid the device holds at once, its waves looping over the slices and
+ * then sharing the fan-out (xcd_swizzle and fanout_waves do not apply; with
+ * graph_steps the captured chunk is one such launch per step).  A barrier
+ * that cannot complete (a grid not wholly resident, as when another
+ * simulation's kernels hold part of the device) gives up after 2 s and the
+ * next synchronising call fails with NC_ERR_STATE.  Results do not depend
+ * on the launch shape.  Default {4, 0, 0, 0, 0, 0}; nc_sim_tune keeps
+ * fanout_waves as it finds it. */
 typedef struct nc_launch_cfg {
   uint32_t waves_per_block;
   uint32_t xcd_swizzle;
   uint32_t graph_steps;
   uint32_t stage_tables;
   uint32_t fanout_waves; /* waves of the fan-out kernel, 64..2^20 (0 = 16384) */
+  uint32_t persistent;   /* 1: one launch per step with a grid barrier */
 } nc_launch_cfg;
 nc_status nc_sim_set_launch(nc_sim* sim, const nc_launch_cfg* cfg);
 nc_status nc_sim_get_launch(const nc_sim* sim, nc_launch_cfg* cfg);
 
-/* Autotuner of the launch shape over the 32 shapes {1, 2, 4, 8 slices per
+/* Autotuner of the launch shape over 38 shapes: {1, 2, 4, 8 slices per
  * workgroup} x {XCD order off/on} x {LDS tables off/on} x {no graph, graph
- * chunks of steps_per_trial steps}, with the current fanout_waves: as
- * nc_sim_tune_list with 3 rounds. */
+ * chunks of steps_per_trial steps}, and the persistent form with {2, 4, 8}
+ * slices per workgroup x {no graph, graph chunks}, with the current
+ * fanout_waves: as nc_sim_tune_list with 3 rounds. */
 nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launch_cfg* chosen);
 
 /* Autotuner over the caller's candidate shapes.  The simulation's state is

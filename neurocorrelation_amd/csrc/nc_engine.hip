@@ -398,6 +398,7 @@ struct nc_sim {
   nc_edge* edges = nullptr;
   uint32_t *pend0 = nullptr, *pend1 = nullptr;
   uint2* pend_cnt = nullptr;
+  uint32_t *bcnt = nullptr, *bent = nullptr;  // arrival buckets: counts, entries
   uint32_t *slice_off = nullptr, *slice_net = nullptr;
   nc_slice_rec* active = nullptr;
   uint32_t *forced_in = nullptr, *forced_bg = nullptr;
@@ -409,7 +410,8 @@ struct nc_sim {
   unsigned long long* ctr = nullptr;
   bool counting = false;
   uint32_t* d_chunk_off = nullptr;  // steps of the current call before the running chunk
-  nc_launch_cfg cfg{4, 0, 0, 0, 0};
+  nc_launch_cfg cfg{4, 0, 0, 0, 0, 0};
+  int resident[9] = {0};  // persistent form: workgroups of 1..8 waves the device holds at once (0 = not known)
   // captured chunk of cfg.graph_steps steps and the launch it was captured for
   hipGraphExec_t graph = nullptr;
   nc_step_args graph_args{};
@@ -494,6 +496,7 @@ nc_step_args base_args(nc_sim* sim) {
   a.raster_cnt = sim->raster_cnt;
   a.raster_cap = sim->raster_cap;
   a.ctr = sim->counting ? sim->ctr : nullptr;
+  a.ctr_rows = (uint32_t)sim->ctr_rows;
   a.t_pre = sim->t_pre;
   a.t_post = sim->t_post;
   a.w_ap = sim->w_ap;
@@ -516,6 +519,8 @@ nc_step_args base_args(nc_sim* sim) {
   a.pend0 = sim->pend0;
   a.pend1 = sim->pend1;
   a.pend_cnt = sim->pend_cnt;
+  a.bcnt = sim->bcnt;
+  a.bent = sim->bent;
   a.fired = sim->fired;
   a.fired_cnt = sim->fired_cnt;
   return a;
@@ -636,6 +641,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   const uint64_t mwords = slots / 32 + 64 * Q;
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
            + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
+           + Q * NC_NB * (4 + 4ull * NC_BCAP)              // arrival buckets
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
            + (n_syn + 64) * sizeof(nc_edge)                // out-edges
            + (Q + 2) * NC_NCTR * 8;                        // counters
@@ -715,6 +721,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->pend0, sim->max_slots * 4), "hipMalloc pend0");
     H(hipMalloc(&sim->pend1, sim->max_slots * 4), "hipMalloc pend1");
     H(hipMalloc(&sim->pend_cnt, Q * sizeof(uint2)), "hipMalloc pend_cnt");
+    H(hipMalloc(&sim->bcnt, Q * NC_NB * 4), "hipMalloc bcnt");
+    H(hipMalloc(&sim->bent, Q * NC_NB * NC_BCAP * 4), "hipMalloc bent");
     H(hipMalloc(&sim->slice_off, Q * 4), "hipMalloc slice_off");
     H(hipMalloc(&sim->slice_net, Q * 4), "hipMalloc slice_net");
     H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
@@ -726,7 +734,9 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
     if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
     sim->ctr_rows = Q + 2;
-    H(hipMalloc(&sim->d_chunk_off, 256), "hipMalloc chunk_off");
+    // word 0: chunk offset; word 32: grid-barrier arrivals of the call;
+    // word 64: a grid barrier timed out (sticky until reported)
+    H(hipMalloc(&sim->d_chunk_off, 1024), "hipMalloc chunk_off");
     H(hipMalloc(&sim->ctr, sim->ctr_rows * NC_NCTR * 8), "hipMalloc ctr");
     // one more, never written: the (never live) network of hole records
     H(hipMalloc(&sim->d_nets, (sim->max_nets + 1ull) * sizeof(nc_netdesc)), "hipMalloc nets");
@@ -740,8 +750,10 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMemsetAsync(sim->fired, 0, (Q + NC_FLISTS * 8ull) * sizeof(uint4), sim->stream), "memset");
     H(hipMemsetAsync(sim->fired_cnt, 0, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
+    H(hipMemsetAsync(sim->bcnt, 0, Q * NC_NB * 4, sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
+    H(hipMemsetAsync(sim->d_chunk_off, 0, 1024, sim->stream), "memset");
     H(hipMemsetAsync(sim->ctr, 0, sim->ctr_rows * NC_NCTR * 8, sim->stream), "memset");
     H(hipMemsetAsync(sim->d_nets, 0, (sim->max_nets + 1ull) * sizeof(nc_netdesc), sim->stream), "memset");
     std::vector<uint32_t> none(Q, NC_SLICE_NONE);
@@ -767,7 +779,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
-                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->bcnt, sim->bent, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p, sim->d_wbuf.p, sim->d_stage.p};
@@ -1029,6 +1041,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   NC_HIP(hipMemcpyAsync(sim->nid + pos0, I.nid.data(), npos * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->out_row + pos0, orow.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * sizeof(uint2), st));
+  NC_HIP(hipMemsetAsync(sim->bcnt + q0 * NC_NB, 0, I.nq * NC_NB * 4ull, st));
   NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
   NC_HIP(hipStreamSynchronize(st));
@@ -1189,6 +1202,26 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
     case 8: fn_neuron = (const void*)k_neuron<8>, fn_fanout = (const void*)k_fanout<8>; break;
     default: fn_neuron = (const void*)k_neuron<4>, fn_fanout = (const void*)k_fanout<4>; break;
   }
+  // The persistent form: one launch per step of a grid the device holds at
+  // once (every workgroup resident, so that the grid barrier can end).
+  const void* fn_step = nullptr;
+  uint32_t pgrid = 0;
+  uint32_t* const bar = sim->d_chunk_off + 32;
+  if (sim->cfg.persistent) {
+    switch (wpb) {
+      case 1: fn_step = (const void*)k_step<1>; break;
+      case 2: fn_step = (const void*)k_step<2>; break;
+      case 8: fn_step = (const void*)k_step<8>; break;
+      default: fn_step = (const void*)k_step<4>; break;
+    }
+    int per_cu = 0, cus = 0;
+    NC_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn_step, (int)(64 * wpb), lds));
+    NC_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, sim->device));
+    pgrid = std::min<uint32_t>((uint32_t)std::max(per_cu, 0) * (uint32_t)std::max(cus, 0), std::max(grid, fan_grid));
+    if (pgrid * wpb < NC_FLISTS) return fail(NC_ERR_STATE, "the device holds too few workgroups for the persistent form");
+    hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, bar, 1u, 0u);
+    NC_HIP(hipGetLastError());
+  }
   hipLaunchKernelGGL(k_set_u32, dim3(1), dim3(64), 0, st, sim->d_chunk_off, 1u, 0u);
   NC_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_zero_strided, dim3(1), dim3(2 * NC_FLISTS), 0, st, sim->fired_cnt, 2u * NC_FLISTS,
@@ -1198,7 +1231,8 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   std::vector<int> timed_kinds;
   auto launch = [&](int kind, const void* fn, uint32_t g, uint32_t block, uint32_t shmem) -> nc_status {
     if (g == 0) return NC_OK;
-    void* args[] = {&a};
+    uint32_t* bar_arg = bar;
+    void* args[] = {&a, &bar_arg};  // k_step takes the barrier words, the others only a
     const bool timed = ((sim->kernel_timing >> kind) & 1) && (sim->timing_step % sim->timing_every) == 0;
     if (timed) {
       // start/stop events carried by the dispatch itself: they time the
@@ -1221,7 +1255,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
           : 0u;
   if (G) {
     a.chunk_i = 0;
-    const uint32_t key[5] = {G, wpb, grid, lds, fan_grid};
+    const uint32_t key[5] = {G, wpb, grid, lds, sim->cfg.persistent ? pgrid | 0x80000000u : fan_grid};
     if (!sim->graph || memcmp(&sim->graph_args, &a, sizeof a) != 0 || memcmp(sim->graph_key, key, sizeof key) != 0) {
       if (sim->graph) {
         (void)hipGraphExecDestroy(sim->graph);
@@ -1231,9 +1265,11 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
       nc_status cs = NC_OK;
       for (uint32_t c = 0; c < G && cs == NC_OK; ++c) {
         a.chunk_i = (int32_t)c;
-        void* args[] = {&a};
-        if (hipLaunchKernel(fn_neuron, dim3(grid), dim3(64 * wpb), args, lds, st) != hipSuccess ||
-            hipLaunchKernel(fn_fanout, dim3(fan_grid), dim3(64 * wpb), args, 0, st) != hipSuccess)
+        uint32_t* bar_arg = bar;
+        void* args[] = {&a, &bar_arg};
+        if (fn_step ? hipLaunchKernel(fn_step, dim3(pgrid), dim3(64 * wpb), args, lds, st) != hipSuccess
+                    : (hipLaunchKernel(fn_neuron, dim3(grid), dim3(64 * wpb), args, lds, st) != hipSuccess ||
+                       hipLaunchKernel(fn_fanout, dim3(fan_grid), dim3(64 * wpb), args, 0, st) != hipSuccess))
           cs = fail(NC_ERR_HIP, "kernel launch during capture failed");
       }
       hipLaunchKernelGGL(k_advance, dim3(1), dim3(64), 0, st, sim->d_chunk_off, G);
@@ -1278,8 +1314,12 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
         hipError_t e = hipLaunchKernel((const void*)k_drive, dim3((a.n_ev + 255) / 256), dim3(256), args, 0, st);
         if (e != hipSuccess) s1 = fail(NC_ERR_HIP, hipGetErrorString(e));
       }
-      if (s1 == NC_OK) s1 = launch(1, fn_neuron, grid, 64 * wpb, lds);
-      if (s1 == NC_OK) s1 = launch(3, fn_fanout, fan_grid, 64 * wpb, 0);
+      if (fn_step) {
+        if (s1 == NC_OK) s1 = launch(1, fn_step, pgrid, 64 * wpb, lds);
+      } else {
+        if (s1 == NC_OK) s1 = launch(1, fn_neuron, grid, 64 * wpb, lds);
+        if (s1 == NC_OK) s1 = launch(3, fn_fanout, fan_grid, 64 * wpb, 0);
+      }
       if (s1 != NC_OK) return s1;
       sim->timing_step++;
     }
@@ -1372,6 +1412,17 @@ extern "C" nc_status nc_push_forced(nc_sim* sim, int32_t net_id, uint32_t n_even
 NC_ABI_CATCH
 
 namespace {
+// After a synchronisation: did a grid barrier of the persistent form time out
+// (a grid whose workgroups were not all resident) since the last check?
+nc_status check_barrier(nc_sim* sim) {
+  uint32_t flag = 0;
+  NC_HIP(hipMemcpy(&flag, sim->d_chunk_off + 64, 4, hipMemcpyDeviceToHost));
+  if (!flag) return NC_OK;
+  NC_HIP(hipMemset(sim->d_chunk_off + 64, 0, 4));
+  return fail(NC_ERR_STATE, "a grid barrier of the persistent launch timed out: the steps since the last check "
+                            "are invalid");
+}
+
 template <class T>
 nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t n) {
   if (!out) return fail(NC_ERR_ARG, "null argument");
@@ -1380,6 +1431,10 @@ nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t 
   NC_HIP(hipMemcpyAsync(tmp.data(), dev + (size_t)r.q0 * 64, tmp.size() * sizeof(T), hipMemcpyDeviceToHost,
                         sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
+  {
+    nc_status cb = check_barrier(sim);
+    if (cb != NC_OK) return cb;
+  }
   for (uint32_t i = 0; i < n; ++i) out[i] = tmp[r.maps->pos_of_id[i]];
   return NC_OK;
 }
@@ -1392,7 +1447,7 @@ nc_status read_slots(nc_sim* sim, NetRec& r, std::vector<nc_slot>* hs) {
   NC_HIP(hipMemcpyAsync(hs->data(), sim->slots + r.slot0, r.nslots * sizeof(nc_slot), hipMemcpyDeviceToHost,
                         sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
-  return NC_OK;
+  return check_barrier(sim);
 }
 }  // namespace
 
@@ -1812,6 +1867,8 @@ nc_status state_save(nc_sim* sim, StateCopy* c) {
       {sim->pend0, sim->max_slots * 4},
       {sim->pend1, sim->max_slots * 4},
       {sim->pend_cnt, Q * sizeof(uint2)},
+      {sim->bcnt, Q * NC_NB * 4},
+      {sim->bent, Q * NC_NB * NC_BCAP * 4},
       {sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)},
       {sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4},
       {sim->forced_in, (P / 32) * 4},
@@ -1853,6 +1910,7 @@ nc_status check_launch(const nc_launch_cfg& c) {
   if (c.stage_tables > 1) return fail(NC_ERR_ARG, "stage_tables must be 0 or 1");
   if (c.fanout_waves && (c.fanout_waves < 64 || c.fanout_waves > (1u << 20)))
     return fail(NC_ERR_ARG, "fanout_waves must be 0 or in [64, 2^20]");
+  if (c.persistent > 1) return fail(NC_ERR_ARG, "persistent must be 0 or 1");
   return NC_OK;
 }
 }  // namespace
@@ -1961,7 +2019,10 @@ extern "C" nc_status nc_sim_tune(nc_sim* sim, uint32_t steps_per_trial, nc_launc
   const uint32_t graphs[2] = {0, std::min<uint32_t>(steps_per_trial, NC_GRING)};
   for (uint32_t g : graphs)
     for (uint32_t w : waves)
-      for (uint32_t x = 0; x < 4; ++x) cands.push_back(nc_launch_cfg{w, x & 1u, g, x >> 1, sim->cfg.fanout_waves});
+      for (uint32_t x = 0; x < 4; ++x) cands.push_back(nc_launch_cfg{w, x & 1u, g, x >> 1, sim->cfg.fanout_waves, 0});
+  // the persistent form (one launch per step): 2, 4 or 8 slices per workgroup
+  for (uint32_t g : graphs)
+    for (uint32_t w : {2u, 4u, 8u}) cands.push_back(nc_launch_cfg{w, 0, g, 0, sim->cfg.fanout_waves, 1});
   return nc_sim_tune_list(sim, cands.data(), (uint32_t)cands.size(), steps_per_trial, 3, chosen, nullptr);
 }
 NC_ABI_CATCH
@@ -1977,7 +2038,7 @@ extern "C" nc_status nc_sim_synchronize(nc_sim* sim) try {
   if (!sim) return fail(NC_ERR_ARG, "null sim");
   NC_HIP(hipSetDevice(sim->device));
   NC_HIP(hipStreamSynchronize(sim->stream));
-  return NC_OK;
+  return check_barrier(sim);
 }
 NC_ABI_CATCH
 

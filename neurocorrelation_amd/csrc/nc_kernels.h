@@ -36,6 +36,14 @@
 // (slot - first slot of the slice) << 8 | (arrival & 255): a list scanned at
 // step s holds no arrival before s nor past s + 255 (delays are 1..255).
 #define NC_PEND_LOG_W 4
+// Spikes with delays up to NC_NB steps go first to an arrival bucket: per
+// slice, one bucket per arrival step modulo NC_NB, of up to NC_BCAP entries
+// (the same entry format).  The step of arrival a reads bucket a % NC_NB of
+// the slice -- exactly its arrivals, no scan of the spikes still in flight --
+// and clears its count.  A spike that finds its bucket full, or whose delay
+// exceeds NC_NB, goes to the pending list of its window.
+#define NC_NB 64u
+#define NC_BCAP 64u
 // Slices with fired lanes go to one of NC_FLISTS lists (by wave index), so
 // that the appends of a step spread over that many counters.
 #define NC_FLISTS 64
@@ -45,7 +53,7 @@
 #define NC_CHUNK_LOG 3
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
-#define NC_NCTR 11  // counter columns (device-side: 2-8, 10)
+#define NC_NCTR 12  // counter columns (device-side: 2-8, 10, 11)
 // Group spike trains are drawn for at most NC_GRING steps ahead, into a ring
 // of that many rows (a call of more steps draws them chunk by chunk).
 #define NC_GRING 1024u
@@ -115,7 +123,8 @@ struct nc_step_args {
   uint32_t* raster;           // (pos, step) pairs
   uint32_t* raster_cnt;
   uint32_t raster_cap;
-  unsigned long long* ctr;  // [grid blocks][NC_NCTR], owned per block
+  unsigned long long* ctr;  // [ctr_rows][NC_NCTR]: workgroup b adds into row b % ctr_rows
+  uint32_t ctr_rows;
   const float* t_pre;
   const float* t_post;
   const float* w_ap;
@@ -133,6 +142,8 @@ struct nc_step_args {
   uint32_t* pend0;         // pending list 0 of every slice, at its slot range
   uint32_t* pend1;         // pending list 1
   uint2* pend_cnt;         // per slice: entries in pending list 0, in list 1
+  uint32_t* bcnt;          // per slice: NC_NB arrival-bucket counts (may exceed NC_BCAP: the excess overflowed)
+  uint32_t* bent;          // per slice: NC_NB buckets of NC_BCAP entries
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
   uint32_t n_ev;
@@ -287,14 +298,13 @@ __global__ void k_drive(nc_step_args a) {
   atomicOr(&m[pos >> 5], 1u << (pos & 31));
 }
 
-// Neuron::run (src/NeuCor.cpp:619-641) for every neuron of the active slices;
-// WPB waves (slices) per workgroup.
+// Neuron::run (src/NeuCor.cpp:619-641) for the 64 neurons of active-slice
+// record `wave` at step ci of the call (one wave of a workgroup of WPB waves;
+// blk is the workgroup's counter row).
 template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
-  extern __shared__ float sh_tab[];  // trace tables: pre then post
-  const uint32_t blk = nc_block_id(a.xcd_swizzle);
+__device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t blk, uint32_t wave, int32_t ci,
+                                                const float* tpre, const float* tpost) {
   const uint32_t wl = threadIdx.x >> 6;
-  const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * WPB + wl);
   const uint32_t lane = threadIdx.x & 63;
   const bool wave_in = wave < a.n_active_slices;
   const uint32_t widx = wave_in ? wave : 0u;
@@ -310,10 +320,11 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   const uint2 pc = a.pend_cnt[q];
   uint32_t sm = a.asum[pos];
   const nc_netdesc nd = a.nets[rec.net];
-  const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;  // step index inside this call
   const bool live = wave_in && ci < nd.remaining;
   const int32_t s = nd.base_step + ci + 1;
-  // The pending list of s's arrival window.
+  // The arrival bucket of step s and the pending list of s's arrival window.
+  uint32_t* const bc = a.bcnt + (size_t)q * NC_NB + ((uint32_t)s & (NC_NB - 1u));
+  uint32_t b_n = live ? *bc : 0u;
   const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
   uint32_t pend_n = plist ? pc.y : pc.x;
   if (!live) {
@@ -322,20 +333,17 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
     sm = 0u;
   }
   uint32_t* const pl = (plist ? a.pend1 : a.pend0) + rec.slot_off;
-  // First 64 pending entries, fetched now and used at the end of the step.
-  uint32_t pe0 = 0u;
+  // The bucket's entries and the first 64 pending entries, fetched now and
+  // used at the end of the step.
+  const bool b_any = b_n != 0u;
+  b_n = min(b_n, NC_BCAP);
+  uint32_t be = 0u, pe0 = 0u;
+  if (lane < b_n) be = a.bent[((size_t)q * NC_NB + ((uint32_t)s & (NC_NB - 1u))) * NC_BCAP + lane];
   if (lane < pend_n) pe0 = pl[lane];
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
   const float lr = nd.lr, pre_f = nd.pre_f, post_f = nd.post_f;
-  if (a.tabs_in_lds) {
-    for (int32_t i = threadIdx.x; i < a.len_pre; i += 64 * WPB) sh_tab[i] = a.t_pre[i];
-    for (int32_t i = threadIdx.x; i < a.len_post; i += 64 * WPB) sh_tab[a.len_pre + i] = a.t_post[i];
-    __syncthreads();
-  }
-  const float* tpre = a.tabs_in_lds ? sh_tab : a.t_pre;
-  const float* tpost = a.tabs_in_lds ? sh_tab + a.len_pre : a.t_post;
   uint32_t n_active = 0, n_arr = 0, n_mwb = 0;
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
@@ -491,30 +499,40 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
     }
   }
 
-  // Spikes of this slice arriving at s leave the pending list and set their
-  // slot's active bit (contributing from s+1).  The list is compacted in
+  // Spikes of this slice arriving at s -- the bucket of step s, then the
+  // pending list of s's window -- set their slot's active bit (contributing
+  // from s+1).  The bucket is emptied; the pending list is compacted in
   // place (write index <= read index).
   // This wave's summary words, merged with the arrival bits below.
   __shared__ uint32_t sh_sum[WPB][64];
   sh_sum[wl][lane] = sm_keep;
+  // entry (slot - first slot of the slice) << 8 | arrival & 255 -> its lane's
+  // mask word bit and summary bit; every lane takes part (the shuffle)
+  auto arrive = [&](bool pred, uint32_t e) {
+    const uint32_t rel = e >> 8;
+    const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
+    const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
+    const uint32_t Gt = (uint32_t)__shfl((int)G, (int)tl);
+    if (pred) {
+      __hip_atomic_fetch_or(a.amask + rec.mask_off + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
+                            __HIP_MEMORY_SCOPE_WORKGROUP);
+      const uint32_t sb = Gt == 1u ? (k >> 5) : (k >> 5) / Gt;
+      __hip_atomic_fetch_or(&sh_sum[wl][tl], 1u << sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+  };
+  if (b_any) {
+    arrive(lane < b_n, be);
+    if (lane == 0) *bc = 0u;
+  }
   if (pend_n) {
     uint32_t kept = 0;
     for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
       const uint32_t t = t0 + lane;
       uint32_t e = pe0;
       if (t0 && t < pend_n) e = pl[t];
-      const bool arrive = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
-      const uint32_t rel = e >> 8;
-      const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
-      const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
-      const uint32_t Gt = (uint32_t)__shfl((int)G, (int)tl);
-      if (arrive) {
-        __hip_atomic_fetch_or(a.amask + rec.mask_off + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_WORKGROUP);
-        const uint32_t sb = Gt == 1u ? (k >> 5) : (k >> 5) / Gt;
-        __hip_atomic_fetch_or(&sh_sum[wl][tl], 1u << sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      const bool keep = t < pend_n && !arrive;
+      const bool arrive_now = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
+      arrive(arrive_now, e);
+      const bool keep = t < pend_n && !arrive_now;
       const unsigned long long km = __ballot(keep);
       const uint32_t to = kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull));
       if (keep && to != t) pl[to] = e;
@@ -547,12 +565,42 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   if (a.ctr) {
     nc_count32(a.ctr, blk, 2, n_active);
     nc_count32(a.ctr, blk, 3, n_arr);
-    if (lane == 0 && fm)
+    if (lane == 0 && fm) {
       atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
+      atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 11], 1ull);
+    }
     nc_count32(a.ctr, blk, 5, n_post);
     nc_count32(a.ctr, blk, 10, n_mwb);
-    if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)pend_n);
+    if (lane == 0 && (pend_n + b_n))
+      atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)(pend_n + b_n));
   }
+}
+
+// The trace tables in LDS when the launch shape asks for it (once per
+// workgroup), else through the caches.
+template <int WPB>
+__device__ __forceinline__ void nc_trace_tables(const nc_step_args& a, const float** tpre, const float** tpost) {
+  extern __shared__ float sh_tab[];  // trace tables: pre then post
+  *tpre = a.t_pre;
+  *tpost = a.t_post;
+  if (a.tabs_in_lds) {
+    for (int32_t i = threadIdx.x; i < a.len_pre; i += 64 * WPB) sh_tab[i] = a.t_pre[i];
+    for (int32_t i = threadIdx.x; i < a.len_post; i += 64 * WPB) sh_tab[a.len_pre + i] = a.t_post[i];
+    __syncthreads();
+    *tpre = sh_tab;
+    *tpost = sh_tab + a.len_pre;
+  }
+}
+
+// The neuron phase of one step for every active slice, one wave per slice,
+// WPB waves per workgroup.
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
+  const uint32_t blk = nc_block_id(a.xcd_swizzle);
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * WPB + (threadIdx.x >> 6));
+  const float *tpre, *tpost;
+  nc_trace_tables<WPB>(a, &tpre, &tpost);
+  nc_neuron_slice<WPB>(a, blk % a.ctr_rows, wave, (int32_t)*a.chunk_off + a.chunk_i, tpre, tpost);
 }
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
@@ -565,31 +613,38 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
                                              uint32_t& n_acc) {
   // a spike still in flight or in its release window (:728)
   if (ed.arr + a.r_rel > s) return;
-  const int32_t arr = s + (int32_t)(ed.tq_d & 0xffu);
+  const uint32_t d = ed.tq_d & 0xffu;
+  const int32_t arr = s + (int32_t)d;
   const uint32_t tq = ed.tq_d >> 8;
-  const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
-  // the list place is taken before the slot is read, so that the two wait together
-  const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
+  // the bucket (or list) place is taken before the slot is read, so that the
+  // two wait together
+  uint32_t* dst = nullptr;
+  if (d <= NC_NB) {
+    const size_t bi = (size_t)tq * NC_NB + ((uint32_t)arr & (NC_NB - 1u));
+    const uint32_t at = atomicAdd(&a.bcnt[bi], 1u);
+    if (at < NC_BCAP) dst = a.bent + bi * NC_BCAP + at;
+  }
+  if (!dst) {
+    const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
+    const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
+    dst = (plist ? a.pend1 : a.pend0) + ed.tbase + at;
+  }
   const nc_slot r = a.slots[ed.j];
   a.slots[ed.j].depol = a.depol_k * r.w;
   a.slots[ed.j].arr_inh = arr * 2 + (r.arr_inh & 1);
   a.edges[e].arr = arr;
-  (plist ? a.pend1 : a.pend0)[ed.tbase + at] = ((ed.j - ed.tbase) << 8) | ((uint32_t)arr & 255u);
+  *dst = ((ed.j - ed.tbase) << 8) | ((uint32_t)arr & 255u);
   ++n_acc;
 }
 
-template <int WPB>
-__global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
-  const uint32_t blk = nc_block_id(a.xcd_swizzle);
-  const uint32_t wave = blk * WPB + (threadIdx.x >> 6);
+__device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t blk, uint32_t wave, uint32_t nwaves,
+                                               int32_t ci) {
   const uint32_t lane = threadIdx.x & 63;
-  const uint32_t nwaves = gridDim.x * WPB;
   uint32_t n_acc = 0, n_out = 0;
   // Wave w takes list w % NC_FLISTS, entries w / NC_FLISTS, + per, ... (64
   // records per load).
   const uint32_t per = nwaves / NC_FLISTS;
   const uint32_t lst = wave % NC_FLISTS, first = wave / NC_FLISTS;
-  const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;
   const uint32_t nf = first < per ? __builtin_amdgcn_readfirstlane(
                                         a.fired_cnt[((((uint32_t)ci & 1u) * NC_FLISTS) + lst) * NC_FCNT_STRIDE])
                                   : 0u;
@@ -649,6 +704,63 @@ __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
     if (lane == 0 && n_out) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 6], (unsigned long long)n_out);
     nc_count32(a.ctr, blk, 7, n_acc);
   }
+}
+
+// The fan-out phase of one step: WPB waves per workgroup share the step's
+// fired-slice records.
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
+  const uint32_t blk = nc_block_id(a.xcd_swizzle);
+  nc_fanout_wave(a, blk % a.ctr_rows, blk * WPB + (threadIdx.x >> 6), gridDim.x * WPB,
+                 (int32_t)*a.chunk_off + a.chunk_i);
+}
+
+// Grid-wide barrier of the persistent step kernel: each workgroup adds one
+// to the counter and waits until it reaches `target` (wrap-safe, the counter
+// runs on across the launches of a call).  The wait is bounded in time:
+// past NC_BARRIER_TICKS of the 100 MHz real-time counter the workgroup flags
+// the failure in *err and goes on, so that every launch ends; the host then
+// reports an error (a grid whose workgroups were not all resident).
+#define NC_BARRIER_TICKS 200000000ull  // 2 s
+#define NC_BARRIER_SPINS (1u << 22)
+__device__ __forceinline__ void nc_grid_barrier(uint32_t* bar, uint32_t target, uint32_t* err) {
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __hip_atomic_fetch_add(bar, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t t0 = __builtin_amdgcn_s_memrealtime();
+    uint32_t spins = 0;
+    while ((int32_t)(__hip_atomic_load(bar, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) - target) < 0) {
+      __builtin_amdgcn_s_sleep(2);
+      // bounded by time and, should the clock not advance, by count
+      if (__builtin_amdgcn_s_memrealtime() - t0 > NC_BARRIER_TICKS || ++spins > NC_BARRIER_SPINS) {
+        __hip_atomic_store(err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// One whole step in one launch of a resident grid (the persistent form):
+// every wave takes active slices w, w + waves, ... through the neuron phase,
+// the grid meets at a barrier, then all waves share the fan-out.  The
+// launch's end orders it before the next step.  bar[0] counts barrier
+// arrivals in this call (zeroed at its start), bar[32] flags a timed-out
+// wait.
+template <int WPB>
+__global__ __launch_bounds__(64 * WPB) void k_step(nc_step_args a, uint32_t* bar) {
+  const uint32_t blk = blockIdx.x % a.ctr_rows;
+  const uint32_t nwaves = gridDim.x * WPB;
+  const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + (threadIdx.x >> 6));
+  const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;
+  const float *tpre, *tpost;
+  nc_trace_tables<WPB>(a, &tpre, &tpost);
+  // every wave of the grid runs the same number of slice passes
+  for (uint32_t b = 0; b < a.n_active_slices; b += nwaves) nc_neuron_slice<WPB>(a, blk, b + w0, ci, tpre, tpost);
+  nc_grid_barrier(bar, ((uint32_t)ci + 1u) * gridDim.x, bar + 32);
+  nc_fanout_wave(a, blk, w0, nwaves, ci);
 }
 
 // Applies the STDP of arrivals at step s_done that k_neuron defers to the

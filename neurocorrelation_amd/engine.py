@@ -214,11 +214,12 @@ class Simulation:
         check(lib().nc_read_counters(self._h, ptr(out, C.c_uint64), len(out)))
         keys = ["neuron_steps", "synapse_steps", "active_slot_steps", "arrivals", "fired",
                 "fired_in_degree", "fired_out_degree", "accepted_spikes", "pending_scanned",
-                "mask_words_read", "mask_words_written"]
+                "mask_words_read", "mask_words_written", "fired_records"]
         return dict(zip(keys, (int(x) for x in out)))
 
-    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0, stage_tables=0, fanout_waves=0):
-        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps, stage_tables, fanout_waves)
+    def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0, stage_tables=0, fanout_waves=0,
+                   persistent=0):
+        cfg = LaunchCfg(waves_per_block, xcd_swizzle, graph_steps, stage_tables, fanout_waves, persistent)
         check(lib().nc_sim_set_launch(self._h, C.byref(cfg)))
 
     def launch(self):
@@ -227,7 +228,7 @@ class Simulation:
         return _cfg_dict(cfg)
 
     def tune(self, steps_per_trial=32):
-        """Autotunes the launch shape over the engine's 32 candidate shapes,
+        """Autotunes the launch shape over the engine's 38 candidate shapes,
         every trial from the same saved state; the simulation is left where
         it was.  Returns the chosen shape."""
         cfg = LaunchCfg()
@@ -240,7 +241,7 @@ class Simulation:
         arr = (LaunchCfg * len(candidates))()
         for i, c in enumerate(candidates):
             arr[i] = LaunchCfg(c.get("waves_per_block", 4), c.get("xcd_swizzle", 0), c.get("graph_steps", 0),
-                               c.get("stage_tables", 0), c.get("fanout_waves", 0))
+                               c.get("stage_tables", 0), c.get("fanout_waves", 0), c.get("persistent", 0))
         ms = np.zeros(len(candidates), np.float32)
         cfg = LaunchCfg()
         check(lib().nc_sim_tune_list(self._h, arr, len(candidates), steps_per_trial, rounds, C.byref(cfg),

@@ -576,3 +576,33 @@ OR_EXPORT float or_ap_value(float dt_since_fire) {
   action_potential(&tmp, 0, dt_since_fire);
   return V;
 }
+
+/* The raw background draws of steps s0+1 .. s0+steps for neurons 0 .. n-1
+ * (the per-step Bernoulli(bg_prob) of or_sim_step, before the refractory
+ * check the simulation applies), as (step, id) pairs ordered by step then id;
+ * returns their number (entries past cap are counted, not written).  For
+ * scripting the same background fires into the reference's own run(). */
+OR_EXPORT uint64_t or_bg_events(uint64_t seed, float bg_prob, uint32_t n, uint32_t s0, uint32_t steps,
+                                uint32_t* ev_steps, uint32_t* ev_ids, uint64_t cap) {
+  uint32_t thr = 0;
+  if (bg_prob > 0.0f) {
+    double th = floor((double)bg_prob * 4294967296.0);
+    thr = th >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)th;
+    if (thr == 0) thr = 1;
+  }
+  const uint64_t bg_seed = seed ^ 0x9E3779B97F4A7C15ull;
+  uint64_t k = 0;
+  for (uint32_t s = s0 + 1; thr && s <= s0 + steps; ++s)
+    for (uint32_t i = 0; i < n; ++i) {
+      uint32_t u[4];
+      draw(bg_seed, s, i, ST_BG, u);
+      if (u[0] < thr) {
+        if (k < cap) {
+          ev_steps[k] = s;
+          ev_ids[k] = i;
+        }
+        ++k;
+      }
+    }
+  return k;
+}

@@ -56,6 +56,8 @@ def lib():
         L.or_trace.argtypes = [C.c_float, C.c_float]
         L.or_ap_value.restype = C.c_float
         L.or_ap_value.argtypes = [C.c_float]
+        L.or_bg_events.restype = C.c_uint64
+        L.or_bg_events.argtypes = [C.c_uint64, C.c_float, C.c_uint32, C.c_uint32, C.c_uint32, P, P, C.c_uint64]
         _L = L
     return _L
 
@@ -201,6 +203,19 @@ def drive_schedule(ids, groups, n_corr_groups, period, p0, amp_corr, amp_ind, se
                 fired.extend(mem)
         per.append(sorted(fired))
     return per
+
+
+def bg_events(seed, bg_prob, n, s0, steps):
+    """Raw background draws of steps s0+1 .. s0+steps (before the refractory
+    check): (steps, ids) ordered by step then id."""
+    cap = max(1024, int(n * steps * bg_prob * 1.5) + 1024)
+    while True:
+        st = np.empty(cap, np.uint32)
+        ids = np.empty(cap, np.uint32)
+        k = lib().or_bg_events(seed, bg_prob, n, s0, steps, _p(st), _p(ids), cap)
+        if k <= cap:
+            return st[:k].copy(), ids[:k].copy()
+        cap = int(k)
 
 
 def trace(decay, x):

@@ -63,11 +63,12 @@ def run_trace(net, steps, **kw):
     return {"raster": raster, "V": V, "w": w, "fire_t": ft, "last_arr": la, "last_fire": lf}
 
 
-def run_bench(net, steps, dt=1.0, inputs=None, workdir=None):
+def run_bench(net, steps, dt=1.0, inputs=None, workdir=None, warmup=0, bg=None):
     """Times the reference's NeuCor::run() over `steps` steps (runAll,
-    runSpeed = dt); inputs fired between runs.  Returns the harness JSON."""
+    runSpeed = dt) after `warmup` untimed ones; inputs (a list of
+    warmup + steps id lists) fired between runs.  Returns the harness JSON."""
     with tempfile.TemporaryDirectory(dir=workdir) as d:
         sp, op = os.path.join(d, "s.bin"), os.path.join(d, "o.json")
-        write_scenario(sp, net, steps, dt=dt, inputs=inputs)
-        subprocess.run([REF_HARNESS, "--bench", sp, op], check=True)
+        write_scenario(sp, net, warmup + steps, dt=dt, inputs=inputs, bg=bg)
+        subprocess.run([REF_HARNESS, "--bench", "--warmup", str(warmup), sp, op], check=True)
         return json.loads(open(op).read())

@@ -133,9 +133,14 @@ int main(int argc, char** argv) {
   }
   if (argc == 4 && strcmp(argv[1], "--wiring") == 0) return dump_wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
   int a = 1;
+  uint32_t warmup = 0;  // --bench: untimed steps before the timed ones
   if (strcmp(argv[a], "--bench") == 0) {
     g_bench = true;
     ++a;
+    if (a + 1 < argc && strcmp(argv[a], "--warmup") == 0) {
+      warmup = (uint32_t)strtoul(argv[a + 1], 0, 10);
+      a += 2;
+    }
   }
   if (argc - a < 2) return 2;
   Scenario sc;
@@ -169,18 +174,28 @@ int main(int argc, char** argv) {
   if (!out) return 1;
   if (g_bench) {
     uint64_t fires = 0;
-    std::vector<float> prev(sc.n);
-    auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t s = 0; s < sc.steps; ++s) {
+    if (warmup > sc.steps) warmup = sc.steps;
+    // background fires scripted as in the trace mode below (the real
+    // rand() is still drawn for every trial of the loop)
+    auto one_step = [&](uint32_t s) {
+      for (uint32_t id : sc.bg_ev[s]) {
+        g_script.push_back(0);
+        g_script.push_back((int)id);
+        g_script.push_back(RAND_MAX);
+      }
       brain.run();
+      g_script.clear();
       for (uint32_t id : sc.in_ev[s]) neu[id].fire();
-    }
+    };
+    for (uint32_t s = 0; s < warmup; ++s) one_step(s);
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t s = warmup; s < sc.steps; ++s) one_step(s);
     double run_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     for (auto& n : neu)
       if (n.lastFire == n.lastFire) ++fires;
-    fprintf(out, "{\"build_s\": %.6f, \"run_s\": %.6f, \"steps\": %u, \"neurons\": %u, \"synapses\": %llu, "
-                 "\"neurons_fired_ever\": %llu}\n",
-            build_s, run_s, sc.steps, sc.n, (unsigned long long)sc.S, (unsigned long long)fires);
+    fprintf(out, "{\"build_s\": %.6f, \"run_s\": %.6f, \"steps\": %u, \"warmup\": %u, \"neurons\": %u, "
+                 "\"synapses\": %llu, \"neurons_fired_ever\": %llu}\n",
+            build_s, run_s, sc.steps - warmup, warmup, sc.n, (unsigned long long)sc.S, (unsigned long long)fires);
     fclose(out);
     return 0;
   }

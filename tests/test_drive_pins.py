@@ -142,3 +142,22 @@ def test_background_draws_are_per_neuron_and_step(oracle_mod):
     nf = m[:-3].sum()
     cond = (m[3:] & m[:-3]).sum() / nf
     assert abs(cond - p) < 5 * math.sqrt(p * (1 - p) / nf), (cond, p)
+
+
+def test_background_event_list_matches_the_simulation(oracle_mod):
+    """The raw background draws that bench.py scripts into the reference's
+    run() are the simulation's: on a network without synapses and a
+    background probability of 1/2, the simulation fires at step s exactly the
+    drawn neurons that are out of their refractory window."""
+    n, steps, p, seed = 50, 300, 0.5, 9
+    ids = np.zeros(0, np.uint32)
+    m = fire_matrix(oracle_mod, n, steps, ids, ids, 0, 0.0, 0.0, 0.0, bg=p, seed=seed)
+    es, ei = oracle_mod.bg_events(seed, p, n, 0, steps)
+    drawn = np.zeros((steps, n), bool)
+    drawn[es.astype(np.int64) - 1, ei.astype(np.int64)] = True
+    assert abs(drawn.mean() - p) < 0.02
+    last = np.full(n, -10)
+    for s in range(steps):
+        want = drawn[s] & (s - last > 2)
+        assert np.array_equal(m[s], want), s
+        last[want] = s

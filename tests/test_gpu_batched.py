@@ -66,7 +66,9 @@ def mix_and_ref():
     return nets, budgets, ref
 
 
-@pytest.mark.parametrize("cfg", [(1, 0, 0), (2, 1, 0, 1, 64), (8, 0, 0), (4, 1, 16, 1, 100), (1, 0, 64, 0, 200), (8, 1, 200, 1)])
+@pytest.mark.parametrize("cfg", [(1, 0, 0), (2, 1, 0, 1, 64), (8, 0, 0), (4, 1, 16, 1, 100), (1, 0, 64, 0, 200),
+                                 (8, 1, 200, 1), (4, 0, 0, 0, 0, 1), (2, 0, 16, 0, 0, 1), (8, 0, 0, 1, 0, 1),
+                                 (1, 0, 64, 0, 0, 1)])
 def test_launch_shapes_give_identical_results(mix_and_ref, cfg):
     nets, budgets, ref = mix_and_ref
     say("launch shape", cfg)

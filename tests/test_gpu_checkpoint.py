@@ -75,3 +75,52 @@ def test_restore_refuses_mismatches():
         b.restore(ib, ckpt[:-4])  # truncated
     a.close()
     b.close()
+
+
+HEADER = 56  # magic, version, n, pad, n_syn (u64), steps, lr, pre_f, post_f, dt, n_events, digest (u64)
+
+
+def test_restore_rejects_out_of_range_steps():
+    """A checkpoint whose last fire or last arrival lies after its own step
+    count, or whose arrival lies outside [1, steps + 255], or with an event of
+    unknown kind, is refused with NC_ERR_ARG before anything is written (the
+    step tables would be indexed out of range); the intact checkpoint then
+    restores."""
+    net, drive, params, _ = W.build("plain-300", seed=4)
+    a = nc.Simulation.for_network(net)
+    ia = a.add(net, params, drive)
+    a.step(40)
+    a.push_forced(ia, np.array([2], np.uint32), np.array([5], np.uint32), np.array([1], np.uint8))
+    good = a.save(ia)
+    a.close()
+    n, S = net.n, net.n_syn
+    steps = int(np.frombuffer(good, np.uint32, 1, 24)[0])
+    assert steps == 40
+    neu, syn, ev = HEADER, HEADER + 12 * n, HEADER + 12 * n + 16 * S
+
+    def patched(off, value):
+        b = bytearray(good)
+        b[off:off + 4] = np.array([value], np.int32).tobytes()
+        return bytes(b)
+
+    bad = [patched(neu + 4, steps + 1),            # neuron 0 fired after the checkpoint's step
+           patched(neu + 4, -(1 << 31) + 1),       # a last fire that would overflow a step difference
+           patched(syn + 8, steps + 3),            # synapse 0 last arrival in the future
+           patched(syn + 12, steps + 256),         # arrival beyond the longest delay
+           patched(syn + 12, -5),                  # arrival before the first step, not "none"
+           patched(ev + 8, 2)]                     # event kind neither input nor background
+    b = nc.Simulation.for_network(net)
+    ib = b.add(net, params, drive)
+    for k, x in enumerate(bad):
+        with pytest.raises(nc.NCError) as e:
+            b.restore(ib, x)
+        assert e.value.code == 1, k
+        assert b.progress(ib)[0] == 0
+    b.restore(ib, patched(syn + 12, -(1 << 30)))  # "no spike" is a valid arrival
+    b.close()
+    c = nc.Simulation.for_network(net)
+    ic = c.add(net, params, drive)
+    c.restore(ic, good)
+    assert c.progress(ic)[0] == 40
+    c.step(10)
+    c.close()

@@ -78,8 +78,10 @@ def test_gpu_matches_reference_fixture(path):
     assert np.array_equal(st["last_fire"], g["ref_last_fire"], equal_nan=True)
 
 
-def run_pair(oracle_mod, net, drive, params, chunks, chunk, raster_cap=1 << 22, seed=0):
-    """Advance the oracle and the GPU together, comparing every chunk."""
+def run_pair(oracle_mod, net, drive, params, chunks, chunk=None, raster_cap=1 << 22, seed=0, launch=None):
+    """Advance the oracle and the GPU together, comparing every chunk
+    (`chunks` chunks of `chunk` steps, or the listed chunk lengths)."""
+    sizes = list(chunks) if chunk is None else [chunk] * chunks
     ex = net.export()
     n, S = len(ex["in_ptr"]) - 1, len(ex["in_src"])
     o = oracle_mod.Sim(ex, dt=params.dt, lr=params.learning_rate, pre_f=params.pre_factor,
@@ -90,8 +92,10 @@ def run_pair(oracle_mod, net, drive, params, chunks, chunk, raster_cap=1 << 22, 
                 drive.seed)
     sim = nc.Simulation.for_network(net, raster_capacity=raster_cap)
     nid = sim.add(net, params, drive)
+    if launch is not None:
+        sim.set_launch(**launch)
     total = 0
-    for c in range(chunks):
+    for c, chunk in enumerate(sizes):
         os_, oi = o.step(chunk, cap=n * chunk)
         sim.step(chunk)
         gs, gi = sim.raster(nid, raster_cap)
@@ -128,18 +132,33 @@ def test_plain300_other_seeds_with_background(oracle_mod, seed):
 
 
 def test_plain20k(oracle_mod):
+    """3000 steps of the 20,000-neuron workload (p ~ 0.27 per step)."""
     net, drive, params, _ = W.build("plain-20k", seed=0)
-    say("plain-20k, 1200 steps")
-    total, _, _ = run_pair(oracle_mod, net, drive, params, 4, 300, raster_cap=1 << 24)
-    assert total > 100_000
+    say("plain-20k, 3000 steps")
+    total, _, _ = run_pair(oracle_mod, net, drive, params, 6, 500, raster_cap=1 << 24)
+    assert total > 300_000
+
+
+@pytest.mark.parametrize("graph_steps", [0, 96])
+def test_calls_longer_than_the_group_ring(oracle_mod, graph_steps):
+    """One call of 2500 steps draws its group spike trains in sub-chunks of
+    at most 1024 steps (960 with graph chunks of 96: whole chunks per
+    sub-chunk, then direct launches for the rest); rasters and state equal
+    the oracle's."""
+    net, drive, params, _ = W.build("plain-300", seed=4)
+    say("plain-300, one 2500-step call, graph_steps", graph_steps)
+    total, _, _ = run_pair(oracle_mod, net, drive, params, [2500, 1100], launch=dict(graph_steps=graph_steps))
+    assert total > 500
 
 
 def test_plain1m_full_size(oracle_mod):
-    """The 10^6-neuron, 10^8-synapse network at full size: 16 steps bit-equal
-    to the oracle, then size-independent invariants over 400 steps."""
+    """The 10^6-neuron, 10^8-synapse network at full size: 144 steps bit-equal
+    to the oracle (delays up to 64 steps arrive; the pending lists' arrival
+    windows flip nine times), then size-independent invariants over 400
+    steps."""
     net, drive, params, _ = W.build("plain-1m", seed=0)
     say("plain-1m built", net.info())
-    total, sim, nid = run_pair(oracle_mod, net, drive, params, 2, 8, raster_cap=1 << 24)
+    total, sim, nid = run_pair(oracle_mod, net, drive, params, [8, 8, 64, 64], raster_cap=1 << 24)
     assert total > 1000
     n, S = net.n, net.n_syn
     before = sim.firings(nid, n).astype(np.int64).sum()
@@ -163,7 +182,7 @@ def test_plain1m_full_size(oracle_mod):
     si, ss = gi[order], gs[order].astype(np.int64)
     same = (si[1:] == si[:-1]) & mask[si[1:]]
     assert np.all(ss[1:][same] - ss[:-1][same] > 2)
-    assert lf.max() <= 416
+    assert lf.max() <= 544
     rate = len(gi) / (n * 400.0)
     say("plain-1m rate", rate)
     assert 0.001 < rate < 0.02
@@ -316,3 +335,35 @@ def test_largest_in_degree(oracle_mod):
     with pytest.raises(nc.NCError) as e:
         sim2.add(too_many, nc.default_params())
     assert e.value.code == 1
+
+
+def test_arrival_bucket_overflow(oracle_mod):
+    """Arrivals beyond a slice's bucket capacity for one step (256 spikes
+    into one 64-neuron slice at the same step) and delays past the bucket
+    ring (100 steps) take the pending lists; potentials, weights and latches
+    equal the oracle's."""
+    srcs, tg = [64, 65, 66, 67], np.arange(64)
+    to = np.concatenate([tg for _ in srcs] + [tg[:8]])
+    frm = np.concatenate([np.full(64, s) for s in srcs] + [np.full(8, 68)])
+    w = np.full(len(to), 0.05, np.float32)
+    delay = np.concatenate([np.full(256, 5), np.full(8, 100)]).astype(np.uint8)
+    net = nc.Network.from_edges(69, to, frm, w, delay)
+    ex = net.export()
+    o = oracle_mod.Sim(ex)
+    sim = nc.Simulation.for_network(net, raster_capacity=1 << 16)
+    nid = sim.add(net, nc.default_params())
+    so = [1, 1, 1, 1, 1, 9, 9, 9, 9]
+    ids = [64, 65, 66, 67, 68, 64, 65, 66, 67]
+    o.push_forced(so, ids, [0] * len(ids))
+    sim.push_forced(nid, so, ids, [0] * len(ids))
+    for k in (4, 3, 10, 100):
+        o.step(k)
+        sim.step(k)
+        st = gpu_state(sim, nid, 69, len(to), 1.0)
+        ost = o.state()
+        assert_bits(st["V"], ost["V"], "V")
+        assert_bits(st["w"], ost["w"], "w")
+        assert np.array_equal(st["fire_t"], ost["fire_t"])
+        assert np.array_equal(st["last_arr"], ost["last_arr"])
+    far = ex["in_src"] == 68
+    assert np.all(ost["last_arr"][~far] == 14.0) and np.all(ost["last_arr"][far] == 101.0)
