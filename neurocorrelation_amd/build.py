@@ -72,6 +72,11 @@ def build_facade():
     _run(["g++"] + CXX_FLAGS + ["-o", exe + ".tmp", os.path.join(fdir, "facade_check.cpp"), obj,
                                 "-L" + HERE, "-lneucor_amd", "-Wl,-rpath,$ORIGIN/..", "-pthread"])
     os.replace(exe + ".tmp", exe)
+    demo = os.path.join(fdir, "renderer_access_demo")
+    _run(["g++"] + CXX_FLAGS + ["-DNC_RENDERER_ACCESS_MAIN", "-o", demo + ".tmp",
+                                os.path.join(fdir, "renderer_access.cpp"), obj, "-L" + HERE, "-lneucor_amd",
+                                "-Wl,-rpath,$ORIGIN/..", "-pthread"])
+    os.replace(demo + ".tmp", demo)
 
 
 if __name__ == "__main__":

@@ -33,6 +33,8 @@ static int wiring(int n, unsigned seed) {
   return 0;
 }
 
+static void dump_mirror(NeuCor& brain);
+
 static int few(long runs) {
   NeuCor brain(0);
   brain.runAll = true;
@@ -53,6 +55,7 @@ static int few(long runs) {
   for (const auto& s : brain.getSynapseSnapshots()) printf("w %zu %zu %a\n", s.fromID, s.toID, s.weight);
   for (const auto& s : brain.getNeuronSnapshots()) printf("v %zu %a\n", s.id, s.potential);
   printf("r %d\n", rand());  // the caller's rand() stream after the run
+  dump_mirror(brain);
   return 0;
 }
 
@@ -89,7 +92,20 @@ class NeuCor_Renderer {
     printf("synapses %zu mismatches %zu latched %zu fired %zu weight_after_set %a\n", i, bad, latched, fired, after);
     return bad == 0 ? 0 : 1;
   }
+
+  // What the renderer reads of each neuron and out-synapse
+  // (src/NeuCor_Renderer.cpp:655-699, :840-866, :1349-1490):
+  // "m FROM TO W AP_FIRETIME LASTARRIVAL" and "l ID LASTFIRE POTENTIAL" (hex).
+  static void dump(NeuCor& brain) {
+    for (auto& neu : brain.neurons) {
+      for (auto& syn : neu.outSynapses)
+        printf("m %zu %zu %a %a %a\n", syn.pN, syn.tN, syn.getWeight(), syn.AP_fireTime, syn.lastSpikeArrival);
+      printf("l %zu %a %a\n", neu.getID(), neu.lastFire, neu.potential());
+    }
+  }
 };
+
+static void dump_mirror(NeuCor& brain) { NeuCor_Renderer::dump(brain); }
 
 int main(int argc, char** argv) {
   try {

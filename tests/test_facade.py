@@ -105,10 +105,14 @@ def test_few_neurons_through_the_facade(oracle_mod):
     out = subprocess.run([_check_bin(), "few", str(runs)], capture_output=True, text=True, timeout=300)
     assert out.returncode == 0, out.stderr
     t = dt = r_next = None
-    w, v = {}, {}
+    w, v, mir, lf = {}, {}, {}, {}
     for line in out.stdout.splitlines():
         f = line.split()
-        if f[0] == "t":
+        if f[0] == "m":
+            mir[(int(f[1]), int(f[2]))] = [np.float32(float.fromhex(x)) for x in f[3:6]]
+        elif f[0] == "l":
+            lf[int(f[1])] = (np.float32(float.fromhex(f[2])), np.float32(float.fromhex(f[3])))
+        elif f[0] == "t":
             t = float.fromhex(f[1])
         elif f[0] == "dt":
             dt = float.fromhex(f[1])
@@ -138,6 +142,15 @@ def test_few_neurons_through_the_facade(oracle_mod):
     st = sim.state()
     assert w[(0, 1)] == st["w"][0] and w[(0, 2)] == st["w"][1]
     assert all(v[i] == st["V"][i] for i in range(3))
+    # the Neuron / Synapse mirror the renderer reads equals the oracle too:
+    # weight, latched arrival (AP_fireTime, 0 = none), last arrival, and each
+    # neuron's last fire and potential
+    for k, key in enumerate([(0, 1), (0, 2)]):
+        mw, mft, mla = mir[key]
+        assert mw == st["w"][k] and mft == st["fire_t"][k] and mla == st["last_arr"][k], (key, mir[key])
+    for i in range(3):
+        assert (lf[i][0] == st["last_fire"][i]) or (np.isnan(lf[i][0]) and np.isnan(st["last_fire"][i]))
+        assert lf[i][1] == st["V"][i]
     # the essay's outcome: the synapse whose target is driven after its
     # source strengthens to 1, the one driven before weakens to about 0
     # (each new pre-before-post pairing and background fire moves it a little)
@@ -160,3 +173,31 @@ def test_renderer_mirror_matches_snapshots():
     # the written weight, possibly moved by plasticity in the one run after it,
     # keeps the inhibitory sign class setWeight gave it
     assert -1.0 <= float.fromhex(vals["weight_after_set"]) <= 0.0
+
+
+def test_renderer_accesses_compile_against_the_facade():
+    """Every expression the reference's renderer (src/NeuCor_Renderer.cpp)
+    and presets (src/main.cpp) apply to the simulation core, repeated in
+    neurocorrelation_amd/facade/renderer_access.cpp (same member names, same
+    result types, cited line by line), compiles against the facade's header.
+    The renderer's own file needs OpenGL, GLFW, ImGui and glm, which this
+    build does not have."""
+    src = os.path.join(ROOT, "neurocorrelation_amd", "facade", "renderer_access.cpp")
+    inc = os.path.join(ROOT, "neurocorrelation_amd", "facade")
+    for extra in ([], ["-DNC_RENDERER_ACCESS_MAIN"]):
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I", inc, src] + extra,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+
+
+@pytest.mark.gpu
+def test_renderer_accesses_run_on_the_facade():
+    """The same accesses run on NeuCor(60) for five frames (alternating
+    wall-clock-scaled and fixed run speeds)."""
+    demo = os.path.join(ROOT, "neurocorrelation_amd", "facade", "renderer_access_demo")
+    if not os.path.exists(demo):
+        pytest.fail("renderer_access_demo is not built (neurocorrelation_amd/build.py)")
+    print("renderer_access_demo", flush=True)
+    out = subprocess.run([demo], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    assert "renderer access ok 1" in out.stdout
