@@ -34,22 +34,28 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
                              lastFire read 4
       40 B per 64 neurons    active-slice record 16, pending counts 8,
                              fired-lane record write 16
-       4 B per mask word     active-slot bit words read (ceil(in-degree/32)
-                             per neuron) and 4 B per word written back
+       4 B per mask word     the arrival-ring word of the step (ceil(in-degree
+           of a neuron       / 32) per neuron), read
+       4 B per ring word     cleared after its arrivals joined the masks
+       4 B per mask word     read (words a summary bit flags) and 4 B per
+                             word written back
       16 B per active slot   slot record {w, depol, lastArrival, arrival}
-      16 B per arrival       arrival STDP (w, lastArrival) 8, arrival bit 8
+       8 B per arrival       arrival STDP: w and lastArrival written
       20 B per in-synapse    post-side STDP of a fired neuron: record read 16,
            of a fired neuron w write 4
       12 B per fired neuron  firings read+write 8, lastFire write 4
-       4 B per pending entry scanned (spikes in flight to the slice in the
-                             list of the step's arrival window)"""
+       4 B per pending entry scanned (spikes whose delays exceed the arrival
+                             ring, in the list of the step's window)"""
     d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree",
-                                               "pending_scanned", "mask_words_read", "mask_words_written")}
-    b = (20.0 * n_neurons + 40.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
-         + 16.0 * d["active_slot_steps"] + 16.0 * d["arrivals"] + 20.0 * d["fired_in_degree"] + 12.0 * d["fired"]
+                                               "pending_scanned", "mask_words_stepped", "mask_words_read",
+                                               "mask_words_written", "ring_words_cleared")}
+    b = (20.0 * n_neurons + 40.0 * n_neurons / 64 + 4.0 * (d["mask_words_stepped"] + d["ring_words_cleared"])
+         + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
+         + 16.0 * d["active_slot_steps"] + 8.0 * d["arrivals"] + 20.0 * d["fired_in_degree"] + 12.0 * d["fired"]
          + 4.0 * d["pending_scanned"])
     return b, {"A": d["active_slot_steps"], "R": d["arrivals"], "F": d["fired"], "F_in": d["fired_in_degree"],
-               "pending": d["pending_scanned"], "mask_words": d["mask_words_read"]}
+               "pending": d["pending_scanned"], "mask_words": d["mask_words_read"],
+               "ring_words": d["mask_words_stepped"], "ring_cleared": d["ring_words_cleared"]}
 
 
 def fanout_kernel_bytes(c0, c1, steps):
@@ -59,11 +65,11 @@ def fanout_kernel_bytes(c0, c1, steps):
        8 B per fired neuron         out-row head {first edge, out-degree}
       16 B per out-edge of a        edge record {target slot, target slice |
            fired neuron             delay, latched arrival, target slice base}
-      32 B per accepted spike       target slot w + arrival word read 8,
+      28 B per accepted spike       target slot w + arrival word read 8,
                                     depol + arrival write 8, edge latch write 4,
-                                    pending entry write 4, pending count RMW 8"""
+                                    arrival-ring word OR (read-modify-write) 8"""
     d = {k: (c1[k] - c0[k]) / steps for k in ("fired_records", "fired", "fired_out_degree", "accepted_spikes")}
-    b = 16.0 * d["fired_records"] + 8.0 * d["fired"] + 16.0 * d["fired_out_degree"] + 32.0 * d["accepted_spikes"]
+    b = 16.0 * d["fired_records"] + 8.0 * d["fired"] + 16.0 * d["fired_out_degree"] + 28.0 * d["accepted_spikes"]
     return b, {"records": d["fired_records"], "F": d["fired"], "F_out": d["fired_out_degree"],
                "accepted": d["accepted_spikes"]}
 

@@ -259,13 +259,17 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * [0] steps*neurons, [1] synapse-steps (sum of S per step),
  * [2] active slot-steps, [3] arrivals, [4] fired neurons,
  * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes,
- * [8] pending-list entries scanned, [9] active-mask words read,
- * [10] active-mask words written back, [11] fired-slice records (64-neuron
- * slices with at least one fired neuron, summed over steps).
+ * [8] pending-list entries scanned (spikes whose delays exceed the
+ * arrival ring), [9] mask words of every stepped neuron, ceil(in-degree /
+ * 32) each (the arrival-ring words a step reads), [10] active-mask words
+ * written back, [11] fired-slice records (64-neuron slices with at least one
+ * fired neuron, summed over steps), [12] active-mask words read (those a
+ * summary bit flags), [13] arrival-ring words cleared (words that held
+ * arrivals).
  * [0], [1] and [9] are kept by the host; the others are device counters that
  * advance only while counting is on (nc_set_counting).
  * Writes the first n (at most NC_NUM_COUNTERS) into out. */
-#define NC_NUM_COUNTERS 12
+#define NC_NUM_COUNTERS 14
 nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
 
 /* Checkpoint of one network's dynamic state.  No counterpart in the
@@ -277,7 +281,25 @@ nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
  * network sits in the simulation), into buf; *size receives the bytes needed
  * (buf = NULL asks for the size only).  nc_sim_restore loads such a state
  * into a network just added (not yet stepped) from the same wiring at the
- * same dt, which then continues exactly as the saved one would have. */
+ * same dt, which then continues exactly as the saved one would have.
+ * nc_sim_restore refuses (NC_ERR_ARG) a checkpoint whose last fires or last
+ * arrivals are neither NC_NEVER_STEP nor in [1, steps done], whose arrivals
+ * are neither NC_NEVER_STEP nor in [1, steps done + 255], or whose events
+ * are of an unknown kind, before it writes anything. */
+/* Byte layout of a checkpoint (little endian, 4-byte fields unless noted):
+ *   header, NC_CKPT_HEADER_BYTES: u32 magic 'NCK1', u32 version 1, u32 n,
+ *     u32 0, u64 synapses, u32 steps done (at NC_CKPT_STEPS_OFFSET),
+ *     f32 learning rate, pre factor, post factor, dt, u32 queued events,
+ *     u64 wiring digest;
+ *   per neuron (id order): f32 potential, i32 last fire step, u32 firings;
+ *   per synapse (nc_net_export order): f32 weight, f32 latched
+ *     depolarisation, i32 last arrival step, i32 arrival step of the latched
+ *     spike (NC_NEVER_STEP: none);
+ *   per queued host event: u32 step offset, u32 neuron id, u32 kind.
+ * A caller may rewrite state fields of a saved checkpoint (steps as
+ * described at nc_sim_restore) but not the header's identity fields. */
+#define NC_CKPT_HEADER_BYTES 56
+#define NC_CKPT_STEPS_OFFSET 24
 nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_t cap, uint64_t* size);
 nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf, uint64_t size);
 

@@ -36,14 +36,13 @@
 // (slot - first slot of the slice) << 8 | (arrival & 255): a list scanned at
 // step s holds no arrival before s nor past s + 255 (delays are 1..255).
 #define NC_PEND_LOG_W 4
-// Spikes with delays up to NC_NB steps go first to an arrival bucket: per
-// slice, one bucket per arrival step modulo NC_NB, of up to NC_BCAP entries
-// (the same entry format).  The step of arrival a reads bucket a % NC_NB of
-// the slice -- exactly its arrivals, no scan of the spikes still in flight --
-// and clears its count.  A spike that finds its bucket full, or whose delay
-// exceeds NC_NB, goes to the pending list of its window.
+// Spikes with delays up to NC_NB steps go to the arrival ring instead: NC_NB
+// rows shaped like the active masks (one bit per slot), row a % NC_NB for the
+// arrivals at step a.  Fan-out sets the slot's bit with an atomic OR that
+// returns nothing; the step of arrival a reads its lanes' words of row
+// a % NC_NB (each lane its own words), ORs them into the active masks and
+// clears them.  Longer delays take the pending lists.
 #define NC_NB 64u
-#define NC_BCAP 64u
 // Slices with fired lanes go to one of NC_FLISTS lists (by wave index), so
 // that the appends of a step spread over that many counters.
 #define NC_FLISTS 64
@@ -53,7 +52,7 @@
 #define NC_CHUNK_LOG 3
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
-#define NC_NCTR 12  // counter columns (device-side: 2-8, 10, 11)
+#define NC_NCTR 14  // counter columns (device-side: 2-8, 10-13)
 // Group spike trains are drawn for at most NC_GRING steps ahead, into a ring
 // of that many rows (a call of more steps draws them chunk by chunk).
 #define NC_GRING 1024u
@@ -83,8 +82,9 @@ struct nc_edge {
   uint32_t tq_d;   // target slice << 8 | delay in steps
   int32_t arr;     // arrival step of the last accepted spike (NC_NEVER: none);
                    // the latch holds until arrival + release window
-  uint32_t tbase;  // first slot of the target slice (where its pending lists start)
+  uint32_t mwb;    // the slot's active-mask word << 5 | its bit (its place in an arrival-ring row)
 };
+#define NC_MAX_MWORDS (1u << 27)  // mask words of a simulation (mwb holds 27 bits of word index)
 
 struct nc_group {
   uint32_t tbl_off;  // first threshold entry
@@ -142,8 +142,8 @@ struct nc_step_args {
   uint32_t* pend0;         // pending list 0 of every slice, at its slot range
   uint32_t* pend1;         // pending list 1
   uint2* pend_cnt;         // per slice: entries in pending list 0, in list 1
-  uint32_t* bcnt;          // per slice: NC_NB arrival-bucket counts (may exceed NC_BCAP: the excess overflowed)
-  uint32_t* bent;          // per slice: NC_NB buckets of NC_BCAP entries
+  uint32_t* ring;          // NC_NB rows of ring_words words (the arrival ring)
+  uint32_t ring_words;     // words per ring row (every mask word of the simulation)
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
   uint32_t n_ev;
@@ -227,16 +227,33 @@ __global__ __launch_bounds__(256) void k_gather_w(const nc_slot* slots, uint64_t
 
 // A network's out-edges from their uploaded network-local form (target slot,
 // target slice << 8 | delay): target slot moved by the network's first slot,
-// target slice by its first slice, the slice's first slot from slice_off
-// (already in place), no spike latched.
+// target slice by its first slice, the slot's active-mask word and bit from
+// the target slice's first slot and first mask word (already in place), no
+// spike latched.
 __global__ __launch_bounds__(256) void k_place_edges(nc_edge* e, const uint2* jq, uint64_t n, uint32_t slot0,
-                                                     uint32_t q0, const uint32_t* slice_off) {
+                                                     uint32_t q0, const uint32_t* slice_off,
+                                                     const uint32_t* slice_moff) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint2 x = jq[i];
     const uint32_t tq_d = x.y + (q0 << 8);
-    e[i] = nc_edge{x.x + slot0, tq_d, NC_NEVER, slice_off[tq_d >> 8]};
+    const uint32_t j = x.x + slot0, tq = tq_d >> 8;
+    // slot j = first slot + (k >> 3) * 512 + lane * 8 + (k & 7) (chunk-major)
+    const uint32_t rel = j - slice_off[tq];
+    const uint32_t lane = (rel >> NC_CHUNK_LOG) & 63u;
+    const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
+    const uint32_t mword = slice_moff[tq] + (k >> 5) * 64u + lane;
+    e[i] = nc_edge{j, tq_d, NC_NEVER, (mword << 5) | (k & 31u)};
   }
+}
+
+// Clears the arrival-ring words of a mask-word range in every row (a network
+// placed where another one's spikes may still have been in flight).
+__global__ __launch_bounds__(256) void k_clear_ring(uint32_t* ring, uint32_t ring_words, uint32_t w0, uint32_t nw) {
+  const uint64_t n = (uint64_t)NC_NB * nw;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    ring[(i / nw) * ring_words + w0 + i % nw] = 0u;
 }
 
 // A network's slot records from their uploaded weights and sign bits (one bit
@@ -322,9 +339,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   const nc_netdesc nd = a.nets[rec.net];
   const bool live = wave_in && ci < nd.remaining;
   const int32_t s = nd.base_step + ci + 1;
-  // The arrival bucket of step s and the pending list of s's arrival window.
-  uint32_t* const bc = a.bcnt + (size_t)q * NC_NB + ((uint32_t)s & (NC_NB - 1u));
-  uint32_t b_n = live ? *bc : 0u;
+  // The pending list of s's arrival window (spikes whose delays exceed the
+  // arrival ring); its first 64 entries are fetched now, used at the end.
   const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
   uint32_t pend_n = plist ? pc.y : pc.x;
   if (!live) {
@@ -333,81 +349,104 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     sm = 0u;
   }
   uint32_t* const pl = (plist ? a.pend1 : a.pend0) + rec.slot_off;
-  // The bucket's entries and the first 64 pending entries, fetched now and
-  // used at the end of the step.
-  const bool b_any = b_n != 0u;
-  b_n = min(b_n, NC_BCAP);
-  uint32_t be = 0u, pe0 = 0u;
-  if (lane < b_n) be = a.bent[((size_t)q * NC_NB + ((uint32_t)s & (NC_NB - 1u))) * NC_BCAP + lane];
+  uint32_t pe0 = 0u;
   if (lane < pend_n) pe0 = pl[lane];
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
   const float lr = nd.lr, pre_f = nd.pre_f, post_f = nd.post_f;
-  uint32_t n_active = 0, n_arr = 0, n_mwb = 0;
+  uint32_t n_active = 0, n_arr = 0, n_mwb = 0, n_mwr = 0, n_ring = 0;
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
   // that arrived at s-R .. s-1) in ascending source order.  Each slot record
   // carries the latched depolarisation and the arrival step, so one fetch
-  // serves the contribution, the arrival STDP and the latch release.
+  // serves the contribution, the arrival STDP and the latch release.  A lane
+  // walks its mask words four at a time, fetched together with its words of
+  // the arrival ring's row of step s: the arrivals at s join the masks after
+  // the charge (they contribute from s+1), and no other lane writes the words.
   const uint32_t nmw = (deg + 31) >> 5;
   const uint32_t G = nmw <= 32 ? 1u : (nmw + 31) >> 5;  // mask words per summary bit
   uint32_t* const mrow = a.amask + rec.mask_off + lane;
-  uint32_t sm_keep = 0;  // summary bits whose words still hold bits after the releases
-  for (uint32_t smi = sm; smi;) {
-    const uint32_t b = (uint32_t)__builtin_ctz(smi);
-    smi &= smi - 1;
-    const uint32_t wend = min((b + 1) * G, nmw);
-    uint32_t any = 0;
-    for (uint32_t w = b * G; w < wend; ++w) {
-      const uint32_t m0 = mrow[w * 64];
-      uint32_t m = m0, bits = m0;
-      while (bits) {
-        // up to four records fetched together, applied in slot order
-        uint32_t jj[4], kb[4];
-        uint32_t cnt = 0;
+  uint32_t* const rrow = a.ring + (size_t)((uint32_t)s & (NC_NB - 1u)) * a.ring_words + rec.mask_off + lane;
+  uint32_t sm_keep = 0;  // summary bits whose words hold bits after the releases and arrivals
+  for (uint32_t w0 = 0; w0 < nmw; w0 += 4) {
+    uint32_t mw[4], rw[4];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          kb[e] = bits ? (uint32_t)__builtin_ctz(bits) : 0u;
-          jj[e] = nc_slot_of(rec.slot_off, lane, 32u * w + kb[e]);
-          if (bits) {
-            bits &= bits - 1;
-            cnt = e + 1;
-          }
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t w = w0 + u;
+      const bool in = w < nmw;
+      const bool flagged = in && ((sm >> (G == 1u ? w : w / G)) & 1u);
+      rw[u] = in ? rrow[w * 64] : 0u;
+      mw[u] = flagged ? mrow[w * 64] : 0u;
+      n_mwr += flagged ? 1u : 0u;
+    }
+    // the active slots of these four words, ascending: bit p of lo (words
+    // w0, w0+1) or of hi (w0+2, w0+3) is in-slot 32 w0 + p
+    unsigned long long lo = ((unsigned long long)mw[1] << 32) | mw[0];
+    unsigned long long hi = ((unsigned long long)mw[3] << 32) | mw[2];
+    unsigned long long klo = lo, khi = hi;  // bits kept after the releases
+    while (lo | hi) {
+      // up to four records fetched together, applied in slot order
+      uint32_t jj[4], pp[4];
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        uint32_t p = 0;
+        if (lo) {
+          p = (uint32_t)__builtin_ctzll(lo);
+          lo &= lo - 1;
+          cnt = e + 1;
+        } else if (hi) {
+          p = 64u + (uint32_t)__builtin_ctzll(hi);
+          hi &= hi - 1;
+          cnt = e + 1;
         }
-        nc_slot r[4];
-#pragma unroll
-        for (int e = 0; e < 4; ++e)
-          if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          if ((uint32_t)e >= cnt) break;
-          const int32_t arr = r[e].arr_inh >> 1;
-          const int32_t o = s - arr;  // in [1, R]: the bit is cleared at o == R
-          const float inc = a.dt * r[e].depol;
-          V = (float)((double)V + ((double)inc * a.c0) * a.e);
-          if (o == 1 && r[e].last_arr != arr) {
-            // the arrival of step s-1 (Synapse::run, :718-726):
-            // lastSpikeArrival = s-1, synapse trace powf(decay, 0) == 1 -> 0,
-            // target trace of lastFire as it stood after step s-1
-            const float tr_arr = nc_lookup(tpost, a.len_post, arr - lf);
-            a.slots[jj[e]].w = nc_stdp(r[e].w, (uint32_t)r[e].arr_inh & 1u, 0.0f, tr_arr, pre_f, post_f, lr);
-            a.slots[jj[e]].last_arr = arr;
-            ++n_arr;
-          }
-          if (o >= a.r_rel) m &= ~(1u << kb[e]);  // latch release (:697)
-        }
-        n_active += cnt;
+        pp[e] = p;
+        jj[e] = nc_slot_of(rec.slot_off, lane, 32u * w0 + p);
       }
-      if (m != m0) {
-        // clear the released bits; arrival bits are OR-ed in below, so both
-        // are read-modify-writes and commute
-        __hip_atomic_fetch_and(&mrow[w * 64], ~(m0 & ~m), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+      nc_slot r[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if ((uint32_t)e >= cnt) break;
+        const int32_t arr = r[e].arr_inh >> 1;
+        const int32_t o = s - arr;  // in [1, R]: the bit is cleared at o == R
+        const float inc = a.dt * r[e].depol;
+        V = (float)((double)V + ((double)inc * a.c0) * a.e);
+        if (o == 1 && r[e].last_arr != arr) {
+          // the arrival of step s-1 (Synapse::run, :718-726):
+          // lastSpikeArrival = s-1, synapse trace powf(decay, 0) == 1 -> 0,
+          // target trace of lastFire as it stood after step s-1
+          const float tr_arr = nc_lookup(tpost, a.len_post, arr - lf);
+          a.slots[jj[e]].w = nc_stdp(r[e].w, (uint32_t)r[e].arr_inh & 1u, 0.0f, tr_arr, pre_f, post_f, lr);
+          a.slots[jj[e]].last_arr = arr;
+          ++n_arr;
+        }
+        if (o >= a.r_rel) {  // latch release (:697)
+          if (pp[e] < 64u) klo &= ~(1ull << pp[e]);
+          else khi &= ~(1ull << (pp[e] - 64u));
+        }
+      }
+      n_active += cnt;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t w = w0 + u;
+      if (w >= nmw) break;
+      const uint32_t kept = (uint32_t)((u < 2 ? klo : khi) >> (32 * (u & 1)));
+      const uint32_t nm = kept | rw[u];
+      if (nm != mw[u]) {
+        mrow[w * 64] = nm;
         ++n_mwb;
       }
-      any |= m;
+      if (rw[u]) {
+        rrow[w * 64] = 0u;
+        ++n_ring;
+      }
+      if (nm) sm_keep |= 1u << (G == 1u ? w : w / G);
     }
-    if (any) sm_keep |= 1u << b;
   }
 
   // charge_passive (src/NeuCor.cpp:677-680).
@@ -499,10 +538,10 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     }
   }
 
-  // Spikes of this slice arriving at s -- the bucket of step s, then the
-  // pending list of s's window -- set their slot's active bit (contributing
-  // from s+1).  The bucket is emptied; the pending list is compacted in
-  // place (write index <= read index).
+  // Spikes of this slice arriving at s from the pending list of s's window
+  // (delays past the arrival ring) set their slot's active bit (contributing
+  // from s+1), in any lane's words; the list is compacted in place (write
+  // index <= read index).
   // This wave's summary words, merged with the arrival bits below.
   __shared__ uint32_t sh_sum[WPB][64];
   sh_sum[wl][lane] = sm_keep;
@@ -520,11 +559,10 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
       __hip_atomic_fetch_or(&sh_sum[wl][tl], 1u << sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     }
   };
-  if (b_any) {
-    arrive(lane < b_n, be);
-    if (lane == 0) *bc = 0u;
-  }
   if (pend_n) {
+    // other lanes' words: after this lane's own stores above
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
     uint32_t kept = 0;
     for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
       const uint32_t t = t0 + lane;
@@ -571,8 +609,9 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     }
     nc_count32(a.ctr, blk, 5, n_post);
     nc_count32(a.ctr, blk, 10, n_mwb);
-    if (lane == 0 && (pend_n + b_n))
-      atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)(pend_n + b_n));
+    if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)pend_n);
+    nc_count32(a.ctr, blk, 12, n_mwr);
+    nc_count32(a.ctr, blk, 13, n_ring);
   }
 }
 
@@ -616,24 +655,22 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   const uint32_t d = ed.tq_d & 0xffu;
   const int32_t arr = s + (int32_t)d;
   const uint32_t tq = ed.tq_d >> 8;
-  // the bucket (or list) place is taken before the slot is read, so that the
-  // two wait together
-  uint32_t* dst = nullptr;
   if (d <= NC_NB) {
-    const size_t bi = (size_t)tq * NC_NB + ((uint32_t)arr & (NC_NB - 1u));
-    const uint32_t at = atomicAdd(&a.bcnt[bi], 1u);
-    if (at < NC_BCAP) dst = a.bent + bi * NC_BCAP + at;
-  }
-  if (!dst) {
+    // the slot's bit in the ring row of its arrival step (no value returned:
+    // the wave does not wait for it)
+    __hip_atomic_fetch_or(a.ring + (size_t)((uint32_t)arr & (NC_NB - 1u)) * a.ring_words + (ed.mwb >> 5),
+                          1u << (ed.mwb & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    // the pending list of its window, at the target slice's slot range
+    const uint32_t tbase = a.slice_off[tq];
     const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
     const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
-    dst = (plist ? a.pend1 : a.pend0) + ed.tbase + at;
+    (plist ? a.pend1 : a.pend0)[tbase + at] = ((ed.j - tbase) << 8) | ((uint32_t)arr & 255u);
   }
   const nc_slot r = a.slots[ed.j];
   a.slots[ed.j].depol = a.depol_k * r.w;
   a.slots[ed.j].arr_inh = arr * 2 + (r.arr_inh & 1);
   a.edges[e].arr = arr;
-  *dst = ((ed.j - ed.tbase) << 8) | ((uint32_t)arr & 255u);
   ++n_acc;
 }
 

@@ -11,12 +11,15 @@
 #include "NeuCor.h"
 
 #include <stdlib.h>
+#include <string.h>
 
 #include <algorithm>
 #include <cassert>
 #include <cmath>
+#include <map>
 #include <stdexcept>
 #include <string>
+#include <utility>
 
 #include "../../include/neucor_amd.h"
 
@@ -273,6 +276,41 @@ void NeuCor::pullWeights() {
 void NeuCor::upload() {
   pullWeights();  // a rebuild keeps the learned weights
   CallerRandStream keep;
+  // A network changed after it has run (createNeuron / createSynapse, as the
+  // reference allows at any time) keeps the dynamic state of its existing
+  // neurons and synapses, as the reference's objects do: potentials, last
+  // fires, firing counts, and every synapse's latched spike and last arrival.
+  struct OldSyn {
+    float depol;
+    int32_t arrival, last_arr;
+  };
+  const std::size_t n_old = uploadedRow.size();
+  std::vector<float> oldV;
+  std::vector<int32_t> oldLf;
+  std::vector<uint32_t> oldFi;
+  std::map<std::pair<std::size_t, std::size_t>, OldSyn> oldSyn;
+  const bool carry = sim && stepsDone > 0 && n_old > 0;
+  const uint32_t oldSteps = (uint32_t)stepsDone;
+  if (carry) {
+    oldV.resize(n_old), oldLf.resize(n_old), oldFi.resize(n_old);
+    check(nc_read_potential(sim, netId, oldV.data(), (uint32_t)n_old), "nc_read_potential");
+    check(nc_read_last_fire(sim, netId, oldLf.data(), (uint32_t)n_old), "nc_read_last_fire");
+    check(nc_read_firings(sim, netId, oldFi.data(), (uint32_t)n_old), "nc_read_firings");
+    const std::size_t S_old = delaySteps.size();
+    std::vector<int32_t> arr(S_old), la(S_old);
+    std::vector<float> dp(S_old);
+    if (S_old)
+      check(nc_read_synapse_state(sim, netId, arr.data(), la.data(), dp.data(), S_old), "nc_read_synapse_state");
+    std::size_t f = 0;
+    for (std::size_t i = 0; i < n_old; ++i)
+      for (std::size_t k = 0; k < uploadedRow[i]; ++k) {
+        const uint64_t e = exportIndex[f++];
+        // nc_read_synapse_state reports a released latch as 0; the checkpoint
+        // takes NC_NEVER_STEP for "no spike" (a released spike is no longer
+        // in flight or in its window, so the two behave alike)
+        oldSyn[{i, neurons[i].outSynapses[k].tN}] = OldSyn{dp[e], arr[e] ? arr[e] : NC_NEVER_STEP, la[e]};
+      }
+  }
   if (sim) {
     nc_sim_destroy(sim);
     sim = nullptr;
@@ -355,6 +393,36 @@ void NeuCor::upload() {
   stepsDone = (uint64_t)std::floor((double)currentTime / stepMs);
   uploadedRow.resize(n);
   for (uint32_t i = 0; i < n; ++i) uploadedRow[i] = neurons[i].outSynapses.size();
+  if (!carry) return;
+  // The carried state, as a checkpoint of the new network (nc_sim_save of the
+  // fresh network gives its header and sizes; the neuron and synapse records
+  // are then overwritten): header of NC_CKPT_HEADER_BYTES, then per neuron
+  // {V, last fire, firings}, per synapse in engine order {w, depol, last
+  // arrival, arrival}, 4 bytes each.
+  uint64_t size = 0;
+  check(nc_sim_save(sim, netId, nullptr, 0, &size), "nc_sim_save");
+  std::vector<unsigned char> buf(size);
+  check(nc_sim_save(sim, netId, buf.data(), size, &size), "nc_sim_save");
+  memcpy(buf.data() + NC_CKPT_STEPS_OFFSET, &oldSteps, 4);
+  unsigned char* nr = buf.data() + NC_CKPT_HEADER_BYTES;
+  for (std::size_t i = 0; i < n_old && i < n; ++i) {
+    memcpy(nr + 12 * i, &oldV[i], 4);
+    memcpy(nr + 12 * i + 4, &oldLf[i], 4);
+    memcpy(nr + 12 * i + 8, &oldFi[i], 4);
+  }
+  unsigned char* sr = nr + 12ull * n;
+  std::size_t f2 = 0;
+  for (uint32_t i = 0; i < n; ++i)
+    for (const Synapse& s : neurons[i].outSynapses) {
+      const uint64_t k = exportIndex[f2++];
+      auto it = oldSyn.find({i, s.tN});
+      if (it == oldSyn.end()) continue;
+      memcpy(sr + 16 * k + 4, &it->second.depol, 4);
+      memcpy(sr + 16 * k + 8, &it->second.last_arr, 4);
+      memcpy(sr + 16 * k + 12, &it->second.arrival, 4);
+    }
+  check(nc_sim_restore(sim, netId, buf.data(), size), "nc_sim_restore");
+  stepsDone = oldSteps;
 }
 
 void NeuCor::pullMirror() {

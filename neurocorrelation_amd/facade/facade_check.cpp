@@ -14,6 +14,10 @@
 //       NeuCor(120) with two inputs for RUNS runs of 1 ms; the Neuron /
 //       Synapse mirror must match the snapshots, and a mirror setWeight must
 //       reach the engine (prints a summary; exit status 1 on a mismatch).
+//   facade_check grow RUNS
+//       the mirror after RUNS runs, then after a createNeuron/createSynapse
+//       and a run too short to step (the engine network rebuilt), then 50
+//       more runs.
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -107,15 +111,42 @@ class NeuCor_Renderer {
 
 static void dump_mirror(NeuCor& brain) { NeuCor_Renderer::dump(brain); }
 
+// NeuCor(120) run RUNS runs of 1 ms; the mirror; then a neuron and a synapse
+// created (the engine network is rebuilt at the next run) and a run too
+// short to reach the next grid step; the mirror again ("after" section);
+// then 50 more runs of 1 ms.
+static int grow(long runs) {
+  srand(3);
+  NeuCor brain(120);
+  brain.runAll = true;
+  brain.runSpeed = 1.0f;
+  float inputs[2] = {40.0f, 25.0f};
+  brain.setInputRateArray(inputs, 2);
+  for (long k = 0; k < runs; ++k) brain.run();
+  printf("before\n");
+  dump_mirror(brain);
+  brain.createNeuron(coord3{9.0f, 9.0f, 9.0f});
+  brain.createSynapse(brain.getNeuronCount() - 1, 0, 0.5f);
+  brain.runSpeed = 1.0f / 1024;
+  brain.run();
+  printf("after\n");
+  dump_mirror(brain);
+  brain.runSpeed = 1.0f;
+  for (int k = 0; k < 50; ++k) brain.run();
+  printf("t %a\n", brain.getTime());
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     if (argc == 4 && strcmp(argv[1], "wiring") == 0) return wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
     if (argc == 3 && strcmp(argv[1], "few") == 0) return few(atol(argv[2]));
     if (argc == 3 && strcmp(argv[1], "mirror") == 0) return NeuCor_Renderer::check(atol(argv[2]));
+    if (argc == 3 && strcmp(argv[1], "grow") == 0) return grow(atol(argv[2]));
   } catch (const std::exception& e) {
     fprintf(stderr, "facade_check: %s\n", e.what());
     return 1;
   }
-  fprintf(stderr, "usage: facade_check wiring N SEED | few RUNS | mirror RUNS\n");
+  fprintf(stderr, "usage: facade_check wiring N SEED | few RUNS | mirror RUNS | grow RUNS\n");
   return 2;
 }

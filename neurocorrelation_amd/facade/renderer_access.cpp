@@ -130,11 +130,13 @@ class NeuCor_Renderer {
 
 // The presets' calls on the public interface (src/main.cpp:80-210).
 static void presets_compile(NeuCor& brain, bool read_back) {
-  std::vector<float> inputs = {10.0f, 20.0f, 30.0f};
-  std::vector<float> inputRadius = {0.8f, 0.8f, 0.8f};
-  std::vector<coord3> inputPositions = {{cosf(0.0f) * 2.0f, sinf(0.0f) * 2.0f, 0.0f},
-                                        {cosf(2.0944f) * 2.0f, sinf(2.0944f) * 2.0f, 0.0f},
-                                        {cosf(4.1888f) * 2.0f, sinf(4.1888f) * 2.0f, 0.0f}};
+  // the rate array is borrowed and re-read by every run() (src/NeuCor.cpp:46-48,
+  // :600): it must outlive the calls, as the presets' session members do
+  static std::vector<float> inputs = {10.0f, 20.0f, 30.0f};
+  static std::vector<float> inputRadius = {0.8f, 0.8f, 0.8f};
+  static std::vector<coord3> inputPositions = {{cosf(0.0f) * 2.0f, sinf(0.0f) * 2.0f, 0.0f},
+                                               {cosf(2.0944f) * 2.0f, sinf(2.0944f) * 2.0f, 0.0f},
+                                               {cosf(4.1888f) * 2.0f, sinf(4.1888f) * 2.0f, 0.0f}};
   // STANDARD (:80-124)
   brain.runSpeed = 4;
   brain.setInputRateArray(inputs.data(), inputs.size(), inputPositions.data(), inputRadius.data());

@@ -198,6 +198,30 @@ def test_renderer_accesses_run_on_the_facade():
     if not os.path.exists(demo):
         pytest.fail("renderer_access_demo is not built (neurocorrelation_amd/build.py)")
     print("renderer_access_demo", flush=True)
-    out = subprocess.run([demo], capture_output=True, text=True, timeout=300)
+    out = subprocess.run([demo], capture_output=True, text=True, timeout=120)
     assert out.returncode == 0, out.stdout + out.stderr
     assert "renderer access ok 1" in out.stdout
+
+
+@pytest.mark.gpu
+def test_network_grown_after_running_keeps_its_state():
+    """createNeuron / createSynapse after run() (which the reference allows at
+    any time) rebuild the engine's network; the existing neurons keep their
+    potential and last fire, the existing synapses their weight, latched
+    spike and last arrival, as the reference's objects do, and the new
+    synapse starts with none."""
+    print("facade_check grow 200", flush=True)
+    out = subprocess.run([_check_bin(), "grow", "200"], capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout + out.stderr
+    sect, lines = None, {"before": [], "after": []}
+    for line in out.stdout.splitlines():
+        if line in ("before", "after"):
+            sect = line
+        elif line[0] in "ml" and sect:
+            lines[sect].append(line)
+    before, after = set(lines["before"]), lines["after"]
+    new = [x for x in after if x not in before]
+    assert len(lines["before"]) > 100 and len(after) == len(lines["before"]) + 2
+    # the new neuron's line and the new synapse 0 -> 120 (no spike: AP_fireTime 0, last arrival -inf)
+    assert sorted(new) == sorted(["m 0 120 0x1p-1 0x0p+0 -inf", "l 120 nan -0x1.18p+6"]), new
+    assert any(x.startswith("m ") and x.split()[4] != "0x0p+0" for x in lines["before"])  # spikes were in flight

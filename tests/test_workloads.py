@@ -54,15 +54,22 @@ def test_neuron_kernel_byte_model():
     spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
     bench = importlib.util.module_from_spec(spec)
     spec.loader.exec_module(bench)
-    keys = ("active_slot_steps", "arrivals", "fired", "fired_in_degree", "pending_scanned", "mask_words_read",
-            "mask_words_written")
-    c0 = {k: 0 for k in keys}
-    c1 = {"active_slot_steps": 100, "arrivals": 10, "fired": 5, "fired_in_degree": 50, "pending_scanned": 200,
-          "mask_words_read": 30, "mask_words_written": 20}
-    b, ev = bench.neuron_kernel_bytes(640, c0, c1, 10)
-    expect = (20 * 640 + 40 * 10 + 4 * (3 + 2) + 16 * 10 + 16 * 1 + 20 * 5 + 12 * 0.5 + 4 * 20)
-    assert abs(b - expect) < 1e-9
-    assert ev["A"] == 10 and ev["F_in"] == 5
+    # bytes per unit, as the docstrings' tables state them
+    per_unit = {"active_slot_steps": 16, "arrivals": 8, "fired": 12, "fired_in_degree": 20, "pending_scanned": 4,
+                "mask_words_stepped": 4, "mask_words_read": 4, "mask_words_written": 4, "ring_words_cleared": 4}
+    zero = {k: 0 for k in per_unit}
+    base, _ = bench.neuron_kernel_bytes(640, zero, zero, 10)
+    assert base == 20 * 640 + 40 * 10  # per neuron and per 64-neuron slice
+    for k, bpu in per_unit.items():
+        c1 = dict(zero, **{k: 30})  # 3 units per step over 10 steps
+        b, ev = bench.neuron_kernel_bytes(640, zero, c1, 10)
+        assert abs(b - base - 3 * bpu) < 1e-9, k
+    fan = {"fired_records": 16, "fired": 8, "fired_out_degree": 16, "accepted_spikes": 28}
+    fz = {k: 0 for k in fan}
+    assert bench.fanout_kernel_bytes(fz, fz, 10)[0] == 0
+    for k, bpu in fan.items():
+        b, _ = bench.fanout_kernel_bytes(fz, dict(fz, **{k: 50}), 10)
+        assert abs(b - 5 * bpu) < 1e-9, k
 
 
 def test_bench_measures_one_gpu_only():
