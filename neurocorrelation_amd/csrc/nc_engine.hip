@@ -399,6 +399,7 @@ struct nc_sim {
   uint32_t *pend0 = nullptr, *pend1 = nullptr;
   uint2* pend_cnt = nullptr;
   uint32_t* ring = nullptr;        // arrival ring: NC_NB rows of max_mwords words
+  uint32_t *id_pos = nullptr, *id_row = nullptr;  // per network, at its positions: neuron id -> local position, first synapse
   uint32_t* slice_moff = nullptr;  // per slice: first active-mask word
   uint32_t *slice_off = nullptr, *slice_net = nullptr;
   nc_slice_rec* active = nullptr;
@@ -621,6 +622,20 @@ nc_status reserve_events(nc_sim* sim) {
   return st;
 }
 
+nc_status check_barrier(nc_sim* sim);
+
+// A network's weights in export order into dev (at least S floats), then the
+// first n of them to the caller's out.
+nc_status export_weights(nc_sim* sim, NetRec& r, float* dev, float* out, uint64_t n) {
+  const uint32_t pos0 = r.q0 * 64;
+  hipLaunchKernelGGL(k_export_w, dim3((r.n + 255) / 256), dim3(256), 0, sim->stream, sim->slots, sim->slice_off,
+                     sim->ndw, sim->id_pos, sim->id_row, pos0, r.q0, r.n, dev);
+  NC_HIP(hipGetLastError());
+  NC_HIP(hipMemcpyAsync(out, dev, n * 4, hipMemcpyDeviceToHost, sim->stream));
+  NC_HIP(hipStreamSynchronize(sim->stream));
+  return check_barrier(sim);
+}
+
 hipEvent_t get_event(nc_sim* sim, size_t i) {
   while (sim->ev_pool.size() <= i) {
     hipEvent_t e;
@@ -726,6 +741,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->pend_cnt, Q * sizeof(uint2)), "hipMalloc pend_cnt");
     H(hipMalloc(&sim->ring, sim->max_mwords * 4 * NC_NB), "hipMalloc ring");
     H(hipMalloc(&sim->slice_moff, Q * 4), "hipMalloc slice_moff");
+    H(hipMalloc(&sim->id_pos, P * 4), "hipMalloc id_pos");
+    H(hipMalloc(&sim->id_row, P * 4), "hipMalloc id_row");
     H(hipMalloc(&sim->slice_off, Q * 4), "hipMalloc slice_off");
     H(hipMalloc(&sim->slice_net, Q * 4), "hipMalloc slice_net");
     H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
@@ -782,7 +799,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
-                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->ring, sim->slice_moff, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->ring, sim->slice_moff, sim->id_pos, sim->id_row, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p, sim->d_wbuf.p, sim->d_stage.p};
@@ -1044,6 +1061,13 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   NC_HIP(hipMemsetAsync(sim->asum + pos0, 0, npos * 4, st));
   NC_HIP(hipMemcpyAsync(sim->ndw + pos0, ndwv.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->nid + pos0, I.nid.data(), npos * 4, hipMemcpyHostToDevice, st));
+  // the export maps: weights are read in export order on the device
+  std::vector<uint32_t> idrow(net->n);
+  for (uint32_t i = 0; i < net->n; ++i) idrow[i] = (uint32_t)M.in_ptr[i];
+  if (net->n) {
+    NC_HIP(hipMemcpyAsync(sim->id_pos + pos0, M.pos_of_id.data(), net->n * 4ull, hipMemcpyHostToDevice, st));
+    NC_HIP(hipMemcpyAsync(sim->id_row + pos0, idrow.data(), net->n * 4ull, hipMemcpyHostToDevice, st));
+  }
   NC_HIP(hipMemcpyAsync(sim->out_row + pos0, orow.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * sizeof(uint2), st));
   if (I.nmwords) {
@@ -1497,17 +1521,10 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   NC_HIP(hipSetDevice(sim->device));
   st = flush_net(sim, *r);
   if (st != NC_OK) return st;
-  if (r->nslots == 0) return NC_OK;
-  st = grow(&sim->d_wbuf, r->nslots, sim->stream);
+  if (r->S == 0 || n == 0) return NC_OK;
+  st = grow(&sim->d_wbuf, r->S, sim->stream);
   if (st != NC_OK) return st;
-  const uint32_t gb = (uint32_t)std::min<uint64_t>((r->nslots + 255) / 256, 4096);
-  hipLaunchKernelGGL(k_gather_w, dim3(gb), dim3(256), 0, sim->stream, sim->slots + r->slot0, r->nslots, sim->d_wbuf.p);
-  NC_HIP(hipGetLastError());
-  std::vector<float> w(r->nslots);
-  NC_HIP(hipMemcpyAsync(w.data(), sim->d_wbuf.p, r->nslots * 4, hipMemcpyDeviceToHost, sim->stream));
-  NC_HIP(hipStreamSynchronize(sim->stream));
-  for (uint64_t k = 0; k < n; ++k) out[k] = w[r->maps->slot_of_syn[k]];
-  return NC_OK;
+  return export_weights(sim, *r, sim->d_wbuf.p, out, n);
 }
 NC_ABI_CATCH
 

@@ -218,6 +218,27 @@ __global__ void k_zero_strided(uint32_t* p, uint32_t n, uint32_t stride) {
   if (threadIdx.x < n) p[threadIdx.x * stride] = 0u;
 }
 
+// Slot index of in-slot k of lane l in a slice whose slots start at slot_off
+// (chunk-major: NC_CHUNK slots per lane per chunk).
+NC_HD uint32_t nc_slot_of(uint32_t slot_off, uint32_t l, uint32_t k) {
+  return slot_off + (k >> NC_CHUNK_LOG) * (64u * NC_CHUNK) + l * NC_CHUNK + (k & (NC_CHUNK - 1u));
+}
+
+// A network's weights in its export order (rows by target id, ascending
+// source): thread i walks neuron i's row, at its position id_pos[pos0 + i]
+// (network-local) and its first synapse id_row[pos0 + i]; the in-degree is
+// the position's; slots are chunk-major inside the slice.
+__global__ __launch_bounds__(256) void k_export_w(const nc_slot* slots, const uint32_t* slice_off, const uint2* ndw,
+                                                  const uint32_t* id_pos, const uint32_t* id_row, uint32_t pos0,
+                                                  uint32_t q0, uint32_t n, float* out) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t p = id_pos[pos0 + i], row = id_row[pos0 + i];
+  const uint32_t deg = ndw[pos0 + p].x & NC_DEG_MASK;
+  const uint32_t so = slice_off[q0 + (p >> 6)], l = p & 63u;
+  for (uint32_t k = 0; k < deg; ++k) out[row + k] = slots[nc_slot_of(so, l, k)].w;
+}
+
 // The weights of a run of slot records, packed 4 bytes per slot (host reads
 // move a quarter of the records' bytes).
 __global__ __launch_bounds__(256) void k_gather_w(const nc_slot* slots, uint64_t n, float* out) {
@@ -272,12 +293,6 @@ __global__ __launch_bounds__(256) void k_place_neurons(float* V, int32_t* last_f
     last_fire[i] = NC_NEVER;
     firings[i] = 0u;
   }
-}
-
-// Slot index of in-slot k of lane l in a slice whose slots start at slot_off
-// (chunk-major: NC_CHUNK slots per lane per chunk).
-NC_HD uint32_t nc_slot_of(uint32_t slot_off, uint32_t l, uint32_t k) {
-  return slot_off + (k >> NC_CHUNK_LOG) * (64u * NC_CHUNK) + l * NC_CHUNK + (k & (NC_CHUNK - 1u));
 }
 
 // Group spike trains of steps c0 .. c0 + gridDim.y - 1 of a call (at most
