@@ -237,6 +237,17 @@ nc_status nc_read_firings(nc_sim* sim, int32_t net_id, uint32_t* out, uint32_t n
 /* Synapse::getWeight / setWeight (src/NeuCor.cpp:572-579). */
 nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, uint64_t n);
 nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* w, uint64_t n);
+
+/* nc_read_weights without any device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * weights are gathered in export order into a static per-GPU scratch array
+ * of 16,777,216 floats (64 MiB, held by every GPU the library runs on until
+ * the process ends) and copied into out.  A network with more synapses than
+ * that fails with NC_ERR_RANGE (nc_read_weights has no such limit).  Calls
+ * that overlap on one GPU (from different threads) share the scratch and
+ * race; calls on one nc_sim are serialised by its threading rule. */
+nc_status nc_read_weights_noalloc(nc_sim* sim, int32_t net_id, float* out, uint64_t n);
 /* Synapse::setWeight on the listed synapses only (indices in export order). */
 nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, const uint64_t* syn, const float* w);
 

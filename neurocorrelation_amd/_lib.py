@@ -90,6 +90,7 @@ SIGNATURES = {
     "nc_read_last_fire": (C.c_int, [P, C.c_int32, i32p, C.c_uint32]),
     "nc_read_firings": (C.c_int, [P, C.c_int32, u32p, C.c_uint32]),
     "nc_read_weights": (C.c_int, [P, C.c_int32, f32p, C.c_uint64]),
+    "nc_read_weights_noalloc": (C.c_int, [P, C.c_int32, f32p, C.c_uint64]),
     "nc_write_weights": (C.c_int, [P, C.c_int32, f32p, C.c_uint64]),
     "nc_set_weights": (C.c_int, [P, C.c_int32, C.c_uint64, u64p, f32p]),
     "nc_read_synapse_state": (C.c_int, [P, C.c_int32, i32p, i32p, f32p, C.c_uint64]),

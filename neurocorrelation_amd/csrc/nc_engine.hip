@@ -24,6 +24,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <map>
 #include <numeric>
 #include <string>
@@ -34,6 +35,33 @@
 #include "nc_kernels.h"
 
 using nc::fail;
+
+// Device scratch of nc_read_weights_noalloc: a file-scope device array, one
+// copy per GPU, held until the process ends.  Host code never passes its name
+// to a kernel (on the host the name is only a stand-in): nc_sim_create looks
+// up each device's address once, with that device current, and the entry
+// point uses that address.  Every call on one GPU shares the array.
+#define NC_SCRATCH_FLOATS (1u << 24)  // 64 MiB: networks of up to 16,777,216 synapses
+static __device__ __align__(256) float scratch[NC_SCRATCH_FLOATS];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kMaxDevices] = {};  // device address of `scratch`, per device
+
+// With `device` current: its copy's address, looked up once.
+nc_status scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(NC_ERR_RANGE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device]) return NC_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(NC_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = static_cast<float*>(ptr);
+  return NC_OK;
+}
+}  // namespace
 
 namespace {
 
@@ -711,6 +739,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
                 std::string(what) + ": " + hipGetErrorString(e));
   };
   H(hipSetDevice(spec->device), "hipSetDevice");
+  if (st == NC_OK) st = scratch_setup(spec->device);  // once per device, with it current
   H(hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking), "hipStreamCreate");
   H(hipEventCreateWithFlags(&sim->upload_done, hipEventDisableTiming), "hipEventCreate");
   sim->max_pos = (spec->max_neurons + 63) / 64 * 64 + 64 * (uint64_t)spec->max_nets;
@@ -1525,6 +1554,27 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   st = grow(&sim->d_wbuf, r->S, sim->stream);
   if (st != NC_OK) return st;
   return export_weights(sim, *r, sim->d_wbuf.p, out, n);
+}
+NC_ABI_CATCH
+
+extern "C" nc_status nc_read_weights_noalloc(nc_sim* sim, int32_t net_id, float* out, uint64_t n) try {
+  NetRec* r;
+  nc_status st = check_net(sim, net_id, &r);
+  if (st != NC_OK) return st;
+  if (!out && n) return fail(NC_ERR_ARG, "null argument");
+  if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
+  if (r->S > NC_SCRATCH_FLOATS)
+    return fail(NC_ERR_RANGE, "the network has more synapses than the static scratch holds (16777216): "
+                              "use nc_read_weights");
+  float* const scr = sim->device < kMaxDevices ? g_scratch_ptr[sim->device] : nullptr;
+  if (!scr) return fail(NC_ERR_STATE, "the static scratch of this device is not set up");
+  NC_HIP(hipSetDevice(sim->device));
+  st = flush_net(sim, *r);
+  if (st != NC_OK) return st;
+  if (r->S == 0 || n == 0) return NC_OK;
+  // the export kernel writes every one of the network's S entries before
+  // the copy reads them: nothing needs clearing first
+  return export_weights(sim, *r, scr, out, n);
 }
 NC_ABI_CATCH
 

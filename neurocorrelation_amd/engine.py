@@ -180,6 +180,13 @@ class Simulation:
         check(lib().nc_read_weights(self._h, net_id, ptr(out, C.c_float), S))
         return out
 
+    def weights_noalloc(self, net_id, S):
+        """weights() through the entry point that allocates no device memory
+        (static per-GPU scratch; networks of at most 16,777,216 synapses)."""
+        out = np.empty(S, np.float32)
+        check(lib().nc_read_weights_noalloc(self._h, net_id, ptr(out, C.c_float), S))
+        return out
+
     def write_weights(self, net_id, w):
         w = np.ascontiguousarray(w, np.float32)
         check(lib().nc_write_weights(self._h, net_id, ptr(w, C.c_float), len(w)))
