@@ -427,6 +427,7 @@ struct nc_sim {
   uint32_t *pend0 = nullptr, *pend1 = nullptr;
   uint2* pend_cnt = nullptr;
   uint32_t* ring = nullptr;        // arrival ring: NC_NB rows of max_mwords words
+  uint2* acache = nullptr;         // active-slot cache: NC_ACACHE rows of P entries
   uint32_t *id_pos = nullptr, *id_row = nullptr;  // per network, at its positions: neuron id -> local position, first synapse
   uint32_t* slice_moff = nullptr;  // per slice: first active-mask word
   uint32_t *slice_off = nullptr, *slice_net = nullptr;
@@ -551,6 +552,8 @@ nc_step_args base_args(nc_sim* sim) {
   a.pend_cnt = sim->pend_cnt;
   a.ring = sim->ring;
   a.ring_words = (uint32_t)sim->max_mwords;
+  a.acache = sim->acache;
+  a.acache_stride = (uint32_t)sim->max_pos;
   a.fired = sim->fired;
   a.fired_cnt = sim->fired_cnt;
   return a;
@@ -617,23 +620,6 @@ nc_status rebuild_active(nc_sim* sim) {
   return NC_OK;
 }
 
-// Pending arrivals of the last completed step get their STDP now.
-nc_status flush_net(nc_sim* sim, NetRec& r) {
-  if (r.steps_done == 0 || r.nq == 0) return NC_OK;
-  // The device list keeps a network's records at act0 until the next
-  // rebuild, whatever else was added or removed since.
-  if (!r.act_valid) {
-    nc_status st = rebuild_active(sim);
-    if (st != NC_OK) return st;
-  }
-  nc_step_args a = base_args(sim);
-  uint32_t blocks = (r.nq + 3) / 4;
-  hipLaunchKernelGGL(k_flush, dim3(blocks), dim3(256), 0, sim->stream, a, r.act0, r.nq,
-                     (int32_t)r.steps_done, r.lr, r.pre_f, r.post_f);
-  NC_HIP(hipGetLastError());
-  return NC_OK;
-}
-
 // Device room for every host event queued on the simulation's networks, so
 // that nc_sim_step, which uploads those due in its steps, never allocates.
 nc_status reserve_events(nc_sim* sim) {
@@ -684,6 +670,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   const uint64_t slots = (n_syn + n_syn / 4 + P * NC_CHUNK + 255) / 256 * 256;  // padding allowance
   const uint64_t mwords = slots / 32 + 64 * Q;
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
+           + P * 8ull * NC_ACACHE                          // active-slot cache
            + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
            + (slots / 32 + 64 * Q) * 4ull * NC_NB + Q * 4     // arrival ring, slice mask offsets
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
@@ -751,7 +738,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
   sim->max_nets = spec->max_nets;
   sim->raster_cap = spec->raster_capacity;
   const uint64_t P = sim->max_pos, Q = P / 64;
-  sim->max_mwords = sim->max_slots / 32 + 64 * Q;
+  // a whole number of 256-byte blocks, so that every arrival-ring row starts on one
+  sim->max_mwords = (sim->max_slots / 32 + 64 * Q + 63) / 64 * 64;
   if (st == NC_OK && sim->max_mwords >= NC_MAX_MWORDS)
     st = fail(NC_ERR_ARG, "capacity too large: more than 2^27 active-mask words (about 4 * 10^9 synapse slots)");
   if (st == NC_OK) {
@@ -769,6 +757,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->pend1, sim->max_slots * 4), "hipMalloc pend1");
     H(hipMalloc(&sim->pend_cnt, Q * sizeof(uint2)), "hipMalloc pend_cnt");
     H(hipMalloc(&sim->ring, sim->max_mwords * 4 * NC_NB), "hipMalloc ring");
+    H(hipMalloc(&sim->acache, P * 8ull * NC_ACACHE), "hipMalloc acache");
     H(hipMalloc(&sim->slice_moff, Q * 4), "hipMalloc slice_moff");
     H(hipMalloc(&sim->id_pos, P * 4), "hipMalloc id_pos");
     H(hipMalloc(&sim->id_row, P * 4), "hipMalloc id_row");
@@ -800,6 +789,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMemsetAsync(sim->fired_cnt, 0, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->ring, 0, sim->max_mwords * 4 * NC_NB, sim->stream), "memset");
+    H(hipMemsetAsync(sim->acache, 0, P * 8ull * NC_ACACHE, sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
     H(hipMemsetAsync(sim->d_chunk_off, 0, 1024, sim->stream), "memset");
@@ -828,7 +818,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
-                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->ring, sim->slice_moff, sim->id_pos, sim->id_row, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->ring, sim->acache, sim->slice_moff, sim->id_pos, sim->id_row, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p, sim->d_wbuf.p, sim->d_stage.p};
@@ -1441,8 +1431,6 @@ extern "C" nc_status nc_set_params(nc_sim* sim, int32_t net_id, float learning_r
   nc_status st = check_net(sim, net_id, &r);
   if (st != NC_OK) return st;
   NC_HIP(hipSetDevice(sim->device));
-  st = flush_net(sim, *r);
-  if (st != NC_OK) return st;
   r->lr = learning_rate;
   r->pre_f = pre_factor;
   r->post_f = post_factor;
@@ -1504,8 +1492,6 @@ nc_status read_pos_array(nc_sim* sim, NetRec& r, const T* dev, T* out, uint32_t 
 
 nc_status read_slots(nc_sim* sim, NetRec& r, std::vector<nc_slot>* hs) {
   NC_HIP(hipSetDevice(sim->device));
-  nc_status st = flush_net(sim, r);
-  if (st != NC_OK) return st;
   hs->resize(r.nslots);
   NC_HIP(hipMemcpyAsync(hs->data(), sim->slots + r.slot0, r.nslots * sizeof(nc_slot), hipMemcpyDeviceToHost,
                         sim->stream));
@@ -1548,8 +1534,6 @@ extern "C" nc_status nc_read_weights(nc_sim* sim, int32_t net_id, float* out, ui
   if (!out) return fail(NC_ERR_ARG, "null argument");
   if (n > r->S) return fail(NC_ERR_RANGE, "more synapses requested than the network has");
   NC_HIP(hipSetDevice(sim->device));
-  st = flush_net(sim, *r);
-  if (st != NC_OK) return st;
   if (r->S == 0 || n == 0) return NC_OK;
   st = grow(&sim->d_wbuf, r->S, sim->stream);
   if (st != NC_OK) return st;
@@ -1569,8 +1553,6 @@ extern "C" nc_status nc_read_weights_noalloc(nc_sim* sim, int32_t net_id, float*
   float* const scr = sim->device < kMaxDevices ? g_scratch_ptr[sim->device] : nullptr;
   if (!scr) return fail(NC_ERR_STATE, "the static scratch of this device is not set up");
   NC_HIP(hipSetDevice(sim->device));
-  st = flush_net(sim, *r);
-  if (st != NC_OK) return st;
   if (r->S == 0 || n == 0) return NC_OK;
   // the export kernel writes every one of the network's S entries before
   // the copy reads them: nothing needs clearing first
@@ -1638,7 +1620,10 @@ extern "C" nc_status nc_read_synapse_state(nc_sim* sim, int32_t net_id, int32_t*
     const uint64_t j = r->maps->slot_of_syn[k];
     const int32_t a = hs[j].arr_inh >> 1;
     if (arrival) arrival[k] = (a != NC_NEVER && s < a + sim->T.r_rel) ? a : 0;
-    if (last_arrival) last_arrival[k] = hs[j].last_arr;
+    // lastSpikeArrival: the latched spike's arrival once it has arrived, else
+    // last_arr (the one before it, which fan-out moved there, or a last
+    // arrival restored without a latched spike)
+    if (last_arrival) last_arrival[k] = (a != NC_NEVER && a <= s) ? a : hs[j].last_arr;
     if (depol) depol[k] = hs[j].depol;
   }
   return NC_OK;
@@ -1746,7 +1731,8 @@ extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_
   for (uint64_t k = 0; k < r->S; ++k) {
     const nc_slot& x = hs[r->maps->slot_of_syn[k]];
     const int32_t arr = x.arr_inh >> 1;
-    memcpy(p, &x.w, 4), memcpy(p + 4, &x.depol, 4), memcpy(p + 8, &x.last_arr, 4), memcpy(p + 12, &arr, 4);
+    const int32_t last = (arr != NC_NEVER && arr <= (int32_t)r->steps_done) ? arr : x.last_arr;  // lastSpikeArrival
+    memcpy(p, &x.w, 4), memcpy(p + 4, &x.depol, 4), memcpy(p + 8, &last, 4), memcpy(p + 12, &arr, 4);
     p += 16;
   }
   for (auto& ev : r->events) {
@@ -1800,6 +1786,9 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   std::vector<nc_slot> hs(r->nslots);
   NC_HIP(hipMemcpy(hs.data(), sim->slots + r->slot0, r->nslots * sizeof(nc_slot), hipMemcpyDeviceToHost));
   std::vector<uint32_t> masks(r->nmwords, 0u);
+  // the active-slot cache: each position's first NC_ACACHE active slots
+  std::vector<uint2> cache((size_t)NC_ACACHE * npos, make_uint2(0u, 0u));
+  std::vector<uint8_t> ncache((size_t)npos, 0u);
   std::vector<std::vector<uint2>> pend(r->nq);
   std::vector<int32_t> edge_arr(r->S);
   for (uint32_t t = 0; t < h.n; ++t) {
@@ -1821,6 +1810,9 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
         const uint64_t word = sim->slice_moff_host[r->q0 + q] + (k >> 5) * 64 + lane - r->mask0;
         masks[word] |= 1u << (k & 31);
         asum[pl] |= 1u << ((k >> 5) / G);
+        uint32_t dbits;
+        memcpy(&dbits, &x.depol, 4);
+        if (ncache[pl] < NC_ACACHE) cache[(size_t)ncache[pl]++ * npos + pl] = make_uint2(dbits, (uint32_t)arr);
       } else if (arr > s) {
         pend[q].push_back(make_uint2((uint32_t)r->slot0 + r->maps->slot_of_syn[syn], (uint32_t)arr));
       }
@@ -1850,6 +1842,9 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipMemcpyAsync(sim->asum + pos0, asum.data(), npos * 4, hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->amask + r->mask0, masks.data(), r->nmwords * 4, hipMemcpyHostToDevice, sq));
+  for (uint32_t e = 0; e < NC_ACACHE; ++e)
+    NC_HIP(hipMemcpyAsync(sim->acache + (size_t)e * sim->max_pos + pos0, cache.data() + (size_t)e * npos,
+                          npos * sizeof(uint2), hipMemcpyHostToDevice, sq));
   // each slice's spikes in flight into the pending list of their arrival
   // window, entries relative to the slice's first slot
   std::vector<uint2> cnt2(r->nq);
@@ -1947,6 +1942,7 @@ nc_status state_save(nc_sim* sim, StateCopy* c) {
       {sim->pend1, sim->max_slots * 4},
       {sim->pend_cnt, Q * sizeof(uint2)},
       {sim->ring, sim->max_mwords * 4 * NC_NB},
+      {sim->acache, P * 8ull * NC_ACACHE},
       {sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)},
       {sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4},
       {sim->forced_in, (P / 32) * 4},

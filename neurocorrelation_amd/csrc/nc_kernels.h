@@ -3,22 +3,25 @@
 // One step s of every active network is, in order:
 //   k_drive   host-scheduled forced fires -> bitmasks       (only when present)
 //   k_neuron  one lane per neuron (one wave per 64-neuron slice):
-//             charge_insynapses over the neuron's active-slot bits,
-//             deferred arrival STDP and latch releases on the same pass,
-//             charge_passive, threshold / forced fire, AP waveform; then the
-//             wave applies post-side STDP to the incoming synapses of each
-//             of its fired lanes, and turns the slice's spikes arriving at s
-//             (the pending list of s's window) into active-slot bits
+//             the slice's spikes arriving at s from its pending list join
+//             the arrival ring's row of s; charge_insynapses over the
+//             neuron's active slots (depolarisation and arrival of the first
+//             NC_ACACHE from the lane's active-slot cache, the rest from
+//             their slot records) with the latch releases, charge_passive,
+//             threshold / forced fire, AP waveform; then the wave applies
+//             post-side STDP to the incoming synapses of each of its fired
+//             lanes, and each lane takes up its arrivals at s from the ring
+//             (pre-side STDP, active bits, cache)
 //   k_fanout  waves sharing the step's records of slices with fired lanes
 //             (64 lists, one record per wave below a cap): Synapse::fire on
 //             the fired lanes' outgoing synapses (source-order latch copy,
-//             out->in slot map), appending accepted spikes to the target
-//             slice's pending list of their arrival's window
+//             out->in slot map), marking accepted spikes in the ring row of
+//             their arrival step, or, past the ring, appending them to the
+//             target slice's pending list of their arrival's window
 // The kernel boundary orders the phases.  Every synapse slot has one writer
 // per phase (its target's wave in k_neuron, its source's wave in k_fanout);
-// the atomics are the appends to the pending lists (per target slice) and
-// to the fired lists (per list), the firing count, and the active bits
-// (workgroup scope, inside one wave's own slice).
+// the atomics are the ring bits, the appends to the pending lists (per
+// target slice) and to the fired lists (per list), and the firing count.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -52,7 +55,14 @@
 #define NC_CHUNK_LOG 3
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
-#define NC_NCTR 14  // counter columns (device-side: 2-8, 10-13)
+#define NC_NCTR 17  // counter columns (device-side: 2-8, 10-16)
+// Each lane keeps (depolarisation, arrival) of its first NC_ACACHE active
+// slots, in slot order, in a per-position cache (entry e of position p at
+// e * stride + p): the charge reads those from the cache, coalesced, and only
+// further active slots from their scattered slot records.  A slot's
+// depolarisation and arrival do not change while it is active (its source
+// cannot fire into it before the latch releases).
+#define NC_ACACHE 4
 // Group spike trains are drawn for at most NC_GRING steps ahead, into a ring
 // of that many rows (a call of more steps draws them chunk by chunk).
 #define NC_GRING 1024u
@@ -144,6 +154,8 @@ struct nc_step_args {
   uint2* pend_cnt;         // per slice: entries in pending list 0, in list 1
   uint32_t* ring;          // NC_NB rows of ring_words words (the arrival ring)
   uint32_t ring_words;     // words per ring row (every mask word of the simulation)
+  uint2* acache;           // active-slot cache: NC_ACACHE rows of acache_stride (depol bits, arrival)
+  uint32_t acache_stride;  // positions of the simulation
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
   uint32_t n_ev;
@@ -168,6 +180,36 @@ __device__ __forceinline__ float nc_stdp(float w, uint32_t inh, float tr_s, floa
   float dw = pre_f * tr_s - post_f * tr_t;
   float wn = w + dw * lr;
   return nc_clamp_w(wn, inh);
+}
+
+// Entry i of a register array of NC_ACACHE cache entries (zero past the end),
+// by selects rather than a scratch-indexed load.
+__device__ __forceinline__ uint2 nc_pick(const uint2 (&c)[NC_ACACHE], uint32_t i) {
+  uint2 v = make_uint2(0u, 0u);
+#pragma unroll
+  for (int e = 0; e < NC_ACACHE; ++e) {
+    v.x = i == (uint32_t)e ? c[e].x : v.x;
+    v.y = i == (uint32_t)e ? c[e].y : v.y;
+  }
+  return v;
+}
+
+// Inserts in-slot k with value v into the ascending list (nk, nv) of n
+// entries, keeping the NC_ACACHE smallest slots.  Slot numbers are distinct;
+// entries past n hold nothing, so whatever the shifts leave there is unused.
+__device__ __forceinline__ void nc_push(uint32_t (&nk)[NC_ACACHE], uint2 (&nv)[NC_ACACHE], uint32_t& n, uint32_t k,
+                                        uint2 v) {
+#pragma unroll
+  for (int e = 0; e < NC_ACACHE; ++e) {
+    const bool here = (uint32_t)e >= n || k < nk[e];
+    const uint32_t tk = nk[e];
+    const uint2 tv = nv[e];
+    nk[e] = here ? k : tk;
+    nv[e] = here ? v : tv;
+    k = here ? tk : k;
+    v = here ? tv : v;
+  }
+  n = n < NC_ACACHE ? n + 1 : n;
 }
 
 __device__ __forceinline__ uint32_t nc_lane() {
@@ -336,7 +378,6 @@ __global__ void k_drive(nc_step_args a) {
 template <int WPB>
 __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t blk, uint32_t wave, int32_t ci,
                                                 const float* tpre, const float* tpost) {
-  const uint32_t wl = threadIdx.x >> 6;
   const uint32_t lane = threadIdx.x & 63;
   const bool wave_in = wave < a.n_active_slices;
   const uint32_t widx = wave_in ? wave : 0u;
@@ -355,7 +396,7 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   const bool live = wave_in && ci < nd.remaining;
   const int32_t s = nd.base_step + ci + 1;
   // The pending list of s's arrival window (spikes whose delays exceed the
-  // arrival ring); its first 64 entries are fetched now, used at the end.
+  // arrival ring).
   const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
   uint32_t pend_n = plist ? pc.y : pc.x;
   if (!live) {
@@ -364,34 +405,70 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     sm = 0u;
   }
   uint32_t* const pl = (plist ? a.pend1 : a.pend0) + rec.slot_off;
-  uint32_t pe0 = 0u;
-  if (lane < pend_n) pe0 = pl[lane];
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
   const float lr = nd.lr, pre_f = nd.pre_f, post_f = nd.post_f;
-  uint32_t n_active = 0, n_arr = 0, n_mwb = 0, n_mwr = 0, n_ring = 0;
-
-  // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
-  // that arrived at s-R .. s-1) in ascending source order.  Each slot record
-  // carries the latched depolarisation and the arrival step, so one fetch
-  // serves the contribution, the arrival STDP and the latch release.  A lane
-  // walks its mask words four at a time, fetched together with its words of
-  // the arrival ring's row of step s: the arrivals at s join the masks after
-  // the charge (they contribute from s+1), and no other lane writes the words.
+  uint32_t n_active = 0, n_arr = 0, n_mwb = 0, n_mwr = 0, n_ring = 0, n_rec = 0, n_wset = 0, n_cw = 0;
   const uint32_t nmw = (deg + 31) >> 5;
   const uint32_t G = nmw <= 32 ? 1u : (nmw + 31) >> 5;  // mask words per summary bit
   uint32_t* const mrow = a.amask + rec.mask_off + lane;
-  uint32_t* const rrow = a.ring + (size_t)((uint32_t)s & (NC_NB - 1u)) * a.ring_words + rec.mask_off + lane;
+  uint32_t* const ring_s = a.ring + (size_t)((uint32_t)s & (NC_NB - 1u)) * a.ring_words + rec.mask_off;
+  uint32_t* const rrow = ring_s + lane;
+  // The lane's cache of (depolarisation, arrival) of its first NC_ACACHE
+  // active slots in slot order, read when the summary says it has any.
+  uint2 ac[NC_ACACHE];
+#pragma unroll
+  for (int e = 0; e < NC_ACACHE; ++e)
+    ac[e] = sm ? a.acache[(size_t)e * a.acache_stride + pos] : make_uint2(0u, 0u);
+
+  // Spikes of this slice arriving at s from the pending list join the ring
+  // row of s (any lane's words: an atomic OR at workgroup scope), where each
+  // lane takes them up with the ring's own arrivals after the fire; the list
+  // is compacted in place (write index <= read index).
+  if (pend_n) {
+    uint32_t kept = 0;
+    for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
+      const uint32_t t = t0 + lane;
+      const uint32_t e = t < pend_n ? pl[t] : 0u;
+      const bool arrive_now = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
+      if (arrive_now) {
+        const uint32_t rel = e >> 8;
+        const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
+        const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
+        __hip_atomic_fetch_or(ring_s + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_WORKGROUP);
+      }
+      const bool keep = t < pend_n && !arrive_now;
+      const unsigned long long km = __ballot(keep);
+      const uint32_t to = kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull));
+      if (keep && to != t) pl[to] = e;
+      kept += (uint32_t)__builtin_popcountll(km);
+    }
+    if (lane == 0) reinterpret_cast<uint32_t*>(&a.pend_cnt[q])[plist] = kept;
+    // the lanes read their ring words below with plain loads: after the ORs
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+
+  // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
+  // that arrived at s-R .. s-1) in ascending source order.  A lane walks its
+  // mask words four at a time; the i-th active slot's depolarisation and
+  // arrival come from the cache while i < NC_ACACHE, else from its slot
+  // record.  The latch is released at o = R (:697).  The kept slots, then the
+  // arrivals of this step, form the next cache.
+  uint32_t nk[NC_ACACHE];  // the next cache: slot numbers and (depol, arrival), ascending
+  uint2 nv[NC_ACACHE];
+  uint32_t ncnt = 0, idx = 0;
+  bool cache_dirty = false;
+  uint32_t kw[4] = {0u, 0u, 0u, 0u};  // kept bits of mask words 0-3
   uint32_t sm_keep = 0;  // summary bits whose words hold bits after the releases and arrivals
   for (uint32_t w0 = 0; w0 < nmw; w0 += 4) {
-    uint32_t mw[4], rw[4];
+    uint32_t mw[4];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const uint32_t w = w0 + u;
-      const bool in = w < nmw;
-      const bool flagged = in && ((sm >> (G == 1u ? w : w / G)) & 1u);
-      rw[u] = in ? rrow[w * 64] : 0u;
+      const bool flagged = w < nmw && ((sm >> (G == 1u ? w : w / G)) & 1u);
       mw[u] = flagged ? mrow[w * 64] : 0u;
       n_mwr += flagged ? 1u : 0u;
     }
@@ -401,8 +478,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     unsigned long long hi = ((unsigned long long)mw[3] << 32) | mw[2];
     unsigned long long klo = lo, khi = hi;  // bits kept after the releases
     while (lo | hi) {
-      // up to four records fetched together, applied in slot order
-      uint32_t jj[4], pp[4];
+      // up to four slots at a time, applied in slot order
+      uint32_t pp[4];
       uint32_t cnt = 0;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
@@ -417,33 +494,32 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
           cnt = e + 1;
         }
         pp[e] = p;
-        jj[e] = nc_slot_of(rec.slot_off, lane, 32u * w0 + p);
       }
-      nc_slot r[4];
+      uint2 dv[4];  // (depol bits, arrival) of each
 #pragma unroll
-      for (int e = 0; e < 4; ++e)
-        if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
+      for (int e = 0; e < 4; ++e) {
+        dv[e] = nc_pick(ac, idx + e);
+        if ((uint32_t)e < cnt && idx + e >= NC_ACACHE) {
+          const nc_slot r = a.slots[nc_slot_of(rec.slot_off, lane, 32u * w0 + pp[e])];
+          dv[e] = make_uint2(__float_as_uint(r.depol), (uint32_t)(r.arr_inh >> 1));
+          ++n_rec;
+        }
+      }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         if ((uint32_t)e >= cnt) break;
-        const int32_t arr = r[e].arr_inh >> 1;
-        const int32_t o = s - arr;  // in [1, R]: the bit is cleared at o == R
-        const float inc = a.dt * r[e].depol;
+        const int32_t o = s - (int32_t)dv[e].y;  // in [1, R]: the bit is cleared at o == R
+        const float inc = a.dt * __uint_as_float(dv[e].x);
         V = (float)((double)V + ((double)inc * a.c0) * a.e);
-        if (o == 1 && r[e].last_arr != arr) {
-          // the arrival of step s-1 (Synapse::run, :718-726):
-          // lastSpikeArrival = s-1, synapse trace powf(decay, 0) == 1 -> 0,
-          // target trace of lastFire as it stood after step s-1
-          const float tr_arr = nc_lookup(tpost, a.len_post, arr - lf);
-          a.slots[jj[e]].w = nc_stdp(r[e].w, (uint32_t)r[e].arr_inh & 1u, 0.0f, tr_arr, pre_f, post_f, lr);
-          a.slots[jj[e]].last_arr = arr;
-          ++n_arr;
-        }
-        if (o >= a.r_rel) {  // latch release (:697)
+        if (o >= a.r_rel) {
           if (pp[e] < 64u) klo &= ~(1ull << pp[e]);
           else khi &= ~(1ull << (pp[e] - 64u));
+          cache_dirty = true;
+        } else {
+          nc_push(nk, nv, ncnt, 32u * w0 + pp[e], dv[e]);
         }
       }
+      idx += cnt;
       n_active += cnt;
     }
 #pragma unroll
@@ -451,16 +527,12 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
       const uint32_t w = w0 + u;
       if (w >= nmw) break;
       const uint32_t kept = (uint32_t)((u < 2 ? klo : khi) >> (32 * (u & 1)));
-      const uint32_t nm = kept | rw[u];
-      if (nm != mw[u]) {
-        mrow[w * 64] = nm;
+      if (w0 == 0) kw[u] = kept;
+      if (kept != mw[u]) {
+        mrow[w * 64] = kept;
         ++n_mwb;
       }
-      if (rw[u]) {
-        rrow[w * 64] = 0u;
-        ++n_ring;
-      }
-      if (nm) sm_keep |= 1u << (G == 1u ? w : w / G);
+      if (kept) sm_keep |= 1u << (G == 1u ? w : w / G);
     }
   }
 
@@ -520,17 +592,12 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     }
   }
   if (fm) {
-    // The row passes below read the fired lanes' slot records from other
-    // lanes; when a fired lane wrote some of them above (arrival STDP), make
-    // those writes visible first.
-    if (__ballot(fire && n_arr != 0)) {
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    }
     // Post-side STDP of each fired lane (the inSynapses loop of Neuron::fire,
     // src/NeuCor.cpp:651-653): target trace powf(decay, 0) == 1 -> 0, synapse
-    // trace of lastSpikeArrival from before this step's arrivals.  The whole
-    // wave works on one fired lane's row at a time.
+    // trace of lastSpikeArrival from before this step's arrivals: the
+    // record's arrival if it came before s, else last_arr (the one before
+    // it, or a last arrival whose latch a restore cleared).  The whole wave
+    // works on one fired lane's row at a time.
     unsigned long long mm = fm;
     while (mm) {
       const uint32_t l = (uint32_t)__builtin_ctzll(mm);
@@ -544,58 +611,91 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
         const nc_slot s0 = a.slots[j0];
         nc_slot s1 = s0;
         if (two) s1 = a.slots[j1];
-        a.slots[j0].w = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - s0.last_arr),
-                                0.0f, pre_f, post_f, lr);
+        const int32_t a0 = s0.arr_inh >> 1, a1 = s1.arr_inh >> 1;
+        // the record's arrival if it is a step in [1, s-1], else last_arr
+        const int32_t la0 = (uint32_t)(a0 - 1) < (uint32_t)(s - 1) ? a0 : s0.last_arr;
+        const int32_t la1 = (uint32_t)(a1 - 1) < (uint32_t)(s - 1) ? a1 : s1.last_arr;
+        a.slots[j0].w = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la0), 0.0f, pre_f,
+                                post_f, lr);
         if (two)
-          a.slots[j1].w = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - s1.last_arr),
-                                  0.0f, pre_f, post_f, lr);
+          a.slots[j1].w = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la1), 0.0f,
+                                  pre_f, post_f, lr);
       }
     }
-  }
-
-  // Spikes of this slice arriving at s from the pending list of s's window
-  // (delays past the arrival ring) set their slot's active bit (contributing
-  // from s+1), in any lane's words; the list is compacted in place (write
-  // index <= read index).
-  // This wave's summary words, merged with the arrival bits below.
-  __shared__ uint32_t sh_sum[WPB][64];
-  sh_sum[wl][lane] = sm_keep;
-  // entry (slot - first slot of the slice) << 8 | arrival & 255 -> its lane's
-  // mask word bit and summary bit; every lane takes part (the shuffle)
-  auto arrive = [&](bool pred, uint32_t e) {
-    const uint32_t rel = e >> 8;
-    const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
-    const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
-    const uint32_t Gt = (uint32_t)__shfl((int)G, (int)tl);
-    if (pred) {
-      __hip_atomic_fetch_or(a.amask + rec.mask_off + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
-                            __HIP_MEMORY_SCOPE_WORKGROUP);
-      const uint32_t sb = Gt == 1u ? (k >> 5) : (k >> 5) / Gt;
-      __hip_atomic_fetch_or(&sh_sum[wl][tl], 1u << sb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    }
-  };
-  if (pend_n) {
-    // other lanes' words: after this lane's own stores above
+    // the arrivals below read records the row passes may have written
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-    uint32_t kept = 0;
-    for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
-      const uint32_t t = t0 + lane;
-      uint32_t e = pe0;
-      if (t0 && t < pend_n) e = pl[t];
-      const bool arrive_now = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
-      arrive(arrive_now, e);
-      const bool keep = t < pend_n && !arrive_now;
-      const unsigned long long km = __ballot(keep);
-      const uint32_t to = kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull));
-      if (keep && to != t) pl[to] = e;
-      kept += (uint32_t)__builtin_popcountll(km);
-    }
-    if (lane == 0) reinterpret_cast<uint32_t*>(&a.pend_cnt[q])[plist] = kept;
   }
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup", "local");
-  const uint32_t sm_new = sh_sum[wl][lane];
-  if (live && sm_new != sm) a.asum[pos] = sm_new;
+
+  // Spikes arriving at s: the lane's words of the ring row of s.  Each one
+  // (Synapse::run, :718-726) takes its pre-side STDP now -- synapse trace
+  // powf(decay, 0) == 1 -> 0, target trace of lastFire after this step; its
+  // record already holds the arrival as lastSpikeArrival -- joins the masks
+  // (contributing from s+1) and the next cache.
+  const float tr_t = nc_lookup(tpost, a.len_post, s - lf);
+  for (uint32_t w0 = 0; w0 < nmw; w0 += 4) {
+    uint32_t rw[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) rw[u] = w0 + u < nmw ? rrow[(w0 + u) * 64] : 0u;
+    unsigned long long lo = ((unsigned long long)rw[1] << 32) | rw[0];
+    unsigned long long hi = ((unsigned long long)rw[3] << 32) | rw[2];
+    while (lo | hi) {
+      uint32_t pp[4], jj[4];
+      uint32_t cnt = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        uint32_t p = 0;
+        if (lo) {
+          p = (uint32_t)__builtin_ctzll(lo);
+          lo &= lo - 1;
+          cnt = e + 1;
+        } else if (hi) {
+          p = 64u + (uint32_t)__builtin_ctzll(hi);
+          hi &= hi - 1;
+          cnt = e + 1;
+        }
+        pp[e] = p;
+        jj[e] = nc_slot_of(rec.slot_off, lane, 32u * w0 + p);
+      }
+      nc_slot r[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if ((uint32_t)e >= cnt) break;
+        if (tr_t != 0.0f) {
+          const float wn = nc_stdp(r[e].w, (uint32_t)r[e].arr_inh & 1u, 0.0f, tr_t, pre_f, post_f, lr);
+          if (__float_as_uint(wn) != __float_as_uint(r[e].w)) {
+            a.slots[jj[e]].w = wn;
+            ++n_wset;
+          }
+        }
+        nc_push(nk, nv, ncnt, 32u * w0 + pp[e], make_uint2(__float_as_uint(r[e].depol), (uint32_t)s));
+        cache_dirty = true;
+      }
+      n_arr += cnt;
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const uint32_t w = w0 + u;
+      if (w >= nmw || !rw[u]) continue;
+      const bool flagged = (sm >> (G == 1u ? w : w / G)) & 1u;
+      const uint32_t base = w0 == 0 ? kw[u] : (flagged ? mrow[w * 64] : 0u);
+      mrow[w * 64] = base | rw[u];
+      rrow[w * 64] = 0u;
+      ++n_mwb;
+      ++n_ring;
+      sm_keep |= 1u << (G == 1u ? w : w / G);
+    }
+  }
+  if (cache_dirty) {
+#pragma unroll
+    for (int e = 0; e < NC_ACACHE; ++e)
+      if ((uint32_t)e < ncnt) a.acache[(size_t)e * a.acache_stride + pos] = nv[e];
+    n_cw += ncnt;
+  }
+  if (live && sm_keep != sm) a.asum[pos] = sm_keep;
   // The neuron's own state is written last (nothing in this kernel reads it
   // back), so that no wait on the way above covers these stores.
   if (valid) {
@@ -627,6 +727,9 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)pend_n);
     nc_count32(a.ctr, blk, 12, n_mwr);
     nc_count32(a.ctr, blk, 13, n_ring);
+    nc_count32(a.ctr, blk, 14, n_rec);
+    nc_count32(a.ctr, blk, 15, n_wset);
+    nc_count32(a.ctr, blk, 16, n_cw);
   }
 }
 
@@ -661,8 +764,8 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
 // fired neuron: each wave takes every (nwaves / 64)-th record of one of the
 // step's 64 fired-slice lists, 64 per load, and the slices one by one; one
 // writer per synapse (its source), latch permitting.  Accepted spikes latch
-// depolarisation and arrival step into the target slot and join the target
-// slice's pending list.
+// depolarisation and arrival step into the target slot and set its bit in the
+// ring row of their arrival step (or join the target slice's pending list).
 __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, const nc_edge& ed, int32_t s,
                                              uint32_t& n_acc) {
   // a spike still in flight or in its release window (:728)
@@ -682,9 +785,16 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
     const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
     (plist ? a.pend1 : a.pend0)[tbase + at] = ((ed.j - tbase) << 8) | ((uint32_t)arr & 255u);
   }
+  // The record's previous arrival (it has arrived: the latch released) moves
+  // to last_arr, which stands for lastSpikeArrival until the new one arrives.
   const nc_slot r = a.slots[ed.j];
-  a.slots[ed.j].depol = a.depol_k * r.w;
-  a.slots[ed.j].arr_inh = arr * 2 + (r.arr_inh & 1);
+  const int32_t prev = r.arr_inh >> 1;
+  nc_slot n;
+  n.w = r.w;
+  n.depol = a.depol_k * r.w;
+  n.last_arr = prev != NC_NEVER ? prev : r.last_arr;
+  n.arr_inh = arr * 2 + (r.arr_inh & 1);
+  a.slots[ed.j] = n;
   a.edges[e].arr = arr;
   ++n_acc;
 }
@@ -813,31 +923,4 @@ __global__ __launch_bounds__(64 * WPB) void k_step(nc_step_args a, uint32_t* bar
   for (uint32_t b = 0; b < a.n_active_slices; b += nwaves) nc_neuron_slice<WPB>(a, blk, b + w0, ci, tpre, tpost);
   nc_grid_barrier(bar, ((uint32_t)ci + 1u) * gridDim.x, bar + 32);
   nc_fanout_wave(a, blk, w0, nwaves, ci);
-}
-
-// Applies the STDP of arrivals at step s_done that k_neuron defers to the
-// next step, so that host reads and parameter changes see eager state.
-__global__ __launch_bounds__(256) void k_flush(nc_step_args a, uint32_t first_rec, uint32_t n_rec, int32_t s_done,
-                                               float lr, float pre_f, float post_f) {
-  const uint32_t wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-  if (wave >= n_rec) return;
-  const uint32_t lane = threadIdx.x & 63;
-  const nc_slice_rec rec = a.active_slices[first_rec + wave];
-  const uint32_t pos = rec.q * 64 + lane;
-  const uint32_t deg = a.ndw[pos].x & NC_DEG_MASK;
-  const int32_t lf = a.last_fire[pos];
-  const uint32_t nmw = (deg + 31) >> 5;
-  for (uint32_t w = 0; w < nmw; ++w) {
-    uint32_t bits = a.amask[rec.mask_off + w * 64 + lane];
-    while (bits) {
-      const uint32_t k = 32 * w + (uint32_t)__builtin_ctz(bits);
-      bits &= bits - 1;
-      const uint32_t j = nc_slot_of(rec.slot_off, lane, k);
-      const nc_slot r = a.slots[j];
-      if ((r.arr_inh >> 1) != s_done || r.last_arr == s_done) continue;
-      const float tr_t = nc_lookup(a.t_post, a.len_post, s_done - lf);
-      a.slots[j].w = nc_stdp(r.w, (uint32_t)r.arr_inh & 1u, 0.0f, tr_t, pre_f, post_f, lr);
-      a.slots[j].last_arr = s_done;
-    }
-  }
 }
