@@ -309,12 +309,16 @@ def bench_batched(args):
 def bench_served(args):
     """The served form: a Poisson stream of jobs (`args.rate` per second for
     `args.window` seconds; network size, degree and step count drawn per job)
-    into one running simulation, admitted by free device room and batched
-    continuously; completion latency from arrival (network construction
-    included) and sims/s.  Beside it, a sample of the same jobs run one
-    after another on the plain path (construction included), whose results
-    must equal the served run's bit for bit."""
-    import neurocorrelation_amd as nc
+    into the native server (nc_server_*): networks built on builder threads
+    (on the GPU with --device-wiring), admitted by free room and batched
+    continuously, evicted and resumed under overload with --evict-after;
+    completion latency from arrival (network construction included) and
+    sims/s.  Beside it, a sample of the same jobs run one after another on the
+    plain path (construction included), whose results must equal the served
+    run's bit for bit; from the sample, a per-job time model (least squares
+    over constant, synapses, synapse-steps) gives the p50/p99 latency of the
+    same arrival stream served one job at a time in arrival order."""
+    from scipy.optimize import nnls
     from neurocorrelation_amd import serve as SV
 
     rng = np.random.default_rng(7)
@@ -324,53 +328,72 @@ def bench_served(args):
                             steps_range=(args.min_job_steps, args.max_job_steps), weights=weights)
     arrivals = np.cumsum(rng.exponential(1.0 / args.rate, size=n_jobs))
     jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
-    probe = nc.Simulation(64, 64)
-    free = probe.free_bytes()
-    probe.close()
-    max_n, max_s = SV.Server.capacity_for(free, fraction=0.5)
-    server = SV.Server(max_n, max_s, max_nets=4096, chunk=args.chunk, lanes=args.lanes)
-    print("[bench] served: %d jobs over %.0fs, device room %d neurons / %d synapses" % (n_jobs, args.window, max_n,
-          max_s), file=sys.stderr, flush=True)
-    done, rest = server.run(jobs, t_limit=args.wall_cap, kernel_every=args.kernel_every,
+    server = SV.Server(0, 0, max_nets=4096, chunk=args.chunk, lanes=args.lanes, builders=args.builders,
+                       device_wiring=args.device_wiring, evict_after=args.evict_after,
+                       overflow_lanes=args.overflow_lanes, hbm_fraction=0.5)
+    print("[bench] served: %d jobs over %.0fs" % (n_jobs, args.window), file=sys.stderr, flush=True)
+    done, rest = server.run(jobs, t_limit=args.wall_cap,
                             log=lambda m: print("[bench] " + m, file=sys.stderr, flush=True))
     server.close()
+    ok = [j for j in done if "digest" in j.result]
     lat = np.array([j.t_done - j.arrival for j in done])
     t_end = max(j.t_done for j in done) if done else 0.0
-    syn_steps = sum(j.n_syn * j.steps for j in done)
+    syn_steps = sum(j.n_syn * j.steps for j in ok)
     # the same jobs alone, one after another, on the plain path: a sample of
     # the completed ones drawn by job index (the same jobs whatever order
     # they completed in)
-    done = sorted(done, key=lambda j: j.idx)
-    sample = sorted(rng.choice(len(done), size=min(args.sample, len(done)), replace=False).tolist()) if done else []
-    seq_t = []
+    ok = sorted(ok, key=lambda j: j.idx)
+    sample = sorted(rng.choice(len(ok), size=min(args.sample, len(ok)), replace=False).tolist()) if ok else []
+    seq_t, feats = [], []
     for i in sample:
-        job = done[i]
+        job = ok[i]
         res, el = SV.run_alone(job.spec)
         assert res == job.result, "job %d differs alone and in the mix" % job.idx
         seq_t.append(el)
-    seq_per_job = float(np.mean(seq_t)) if seq_t else None
+        feats.append([1.0, job.n_syn, job.n_syn * job.steps])
+    seq = None
+    if len(seq_t) >= 3:
+        coef, _ = nnls(np.array(feats), np.array(seq_t))
+        # every job of the stream (sizes from the served run; unfinished jobs
+        # sized from their spec's bulk degree) served alone in arrival order
+        by = {j.idx: j for j in ok}
+        t_job = []
+        for j in sorted(jobs, key=lambda j: j.arrival):
+            syn = by[j.idx].n_syn if j.idx in by else j.spec["wiring"]["n_neurons"] * j.spec["k"]
+            t_job.append(coef[0] + coef[1] * syn + coef[2] * syn * j.steps)
+        arr = np.array(sorted(j.arrival for j in jobs))
+        fin = np.empty_like(arr)
+        c = 0.0
+        for k, (a_, t_) in enumerate(zip(arr, t_job)):
+            c = max(a_, c) + t_
+            fin[k] = c
+        slat = fin - arr
+        seq = {"sample": len(sample), "mean_s_per_job": float(np.mean(seq_t)), "sims_per_s": 1.0 / float(np.mean(t_job)),
+               "identical_results": True,
+               "model_s": {"per_job": coef[0], "per_synapse": coef[1], "per_synapse_step": coef[2]},
+               "latency_s_modelled": {"p50": float(np.percentile(slat, 50)), "p99": float(np.percentile(slat, 99))}}
     out = {
         "metric": "served: sims/s and p50/p99 completion latency vs sequential plain runs",
-        "value": len(done) / t_end if t_end else None, "unit": "sims/s", "n_gpus": 1, "steps": None,
+        "value": len(ok) / t_end if t_end else None, "unit": "sims/s", "n_gpus": 1, "steps": None,
         "warmup": 0, "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32", "data": "synthetic",
         "config": {"workload": "served", "rate_per_s": args.rate, "window_s": args.window, "jobs": n_jobs,
                    "neurons": [300, args.max_neurons], "synapses_per_neuron": [20, 100],
                    "job_steps": [args.min_job_steps, args.max_job_steps], "chunk": args.chunk, "lanes": args.lanes,
+                   "builders": args.builders, "device_wiring": bool(args.device_wiring),
+                   "evict_after_s": args.evict_after, "overflow_lanes": args.overflow_lanes,
                    "weights": {"w_min": weights[0], "w_max": weights[1], "inhib_frac": weights[2]},
                    "wall_cap_s": args.wall_cap},
-        "completed": len(done), "unfinished": rest["unfinished"],
+        "completed": len(ok), "refused": len(done) - len(ok), "unfinished": rest["unfinished"],
         "latency_s": {"p50": float(np.percentile(lat, 50)) if len(lat) else None,
                       "p99": float(np.percentile(lat, 99)) if len(lat) else None,
                       "max": float(lat.max()) if len(lat) else None},
+        "build_s": {"p50": float(np.percentile([j.t_built - j.arrival for j in ok], 50)) if ok else None},
         "synapse_updates_per_s": syn_steps / t_end if t_end else None,
-        "sequential_plain": {"sample": len(sample), "mean_s_per_job": seq_per_job,
-                             "sims_per_s": 1.0 / seq_per_job if seq_per_job else None,
-                             "identical_results": True},
-        "speedup_vs_sequential": (len(done) / t_end) * seq_per_job if (t_end and seq_per_job) else None,
+        "sequential_plain": seq,
+        "speedup_vs_sequential": ((len(ok) / t_end) / seq["sims_per_s"]) if (t_end and seq) else None,
         "serving_loop": rest["loop"],
-        "mean_rate_per_step": (sum(j.result["spikes"] for j in done if "spikes" in j.result) /
-                               max(1, sum(j.n * j.steps for j in done if "spikes" in j.result))),
+        "mean_rate_per_step": (sum(j.result["spikes"] for j in ok) / max(1, sum(j.n * j.steps for j in ok))),
     }
     print(json.dumps(out), flush=True)
 
@@ -403,8 +426,12 @@ def main():
     ap.add_argument("--wall-cap", type=float, default=900.0, help="served: stop serving after this many seconds")
     ap.add_argument("--lanes", type=int, default=2,
                     help="served: simulations the GPU alternates between (host work of one overlaps the other's chunk)")
-    ap.add_argument("--kernel-every", type=int, default=0,
-                    help="served: time every K-th step's kernels with HIP events (0: none)")
+    ap.add_argument("--builders", type=int, default=8, help="served: host threads building job networks")
+    ap.add_argument("--device-wiring", action="store_true", help="served: build job networks on the GPU")
+    ap.add_argument("--evict-after", type=float, default=0.0,
+                    help="served: seconds a job waits for room before it evicts the job with most steps left")
+    ap.add_argument("--overflow-lanes", type=int, default=0,
+                    help="served: extra lanes made from free HBM while jobs wait")
     ap.add_argument("--runaway", action="store_true",
                     help="served: all-excitatory weights in [0.2, 0.8) (jobs run at p ~ 0.27 instead of ~0.0025)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -466,14 +493,22 @@ def main():
     launch_used = sim.launch()
     sim.set_kernel_timing(0)
     print("[bench] timed %d steps in %.3fs" % (args.steps, elapsed), file=sys.stderr, flush=True)
+    sim.close()
     if not args.no_kernel_timing:
+        # the same steps on a fresh simulation without the events (the step is
+        # deterministic: the same activity), for what the events cost
+        sim = nc.Simulation.for_network(net)
+        sim.add(net, params, drive)
+        sim.set_launch(**launch)
+        run(sim, args.warmup)
+        sim.synchronize()
         t1 = time.perf_counter()
         run(sim, args.steps)
         sim.synchronize()
-        print("[bench] next %d steps without kernel events: %.2f us/step (timed run: %.2f)" %
+        print("[bench] the same %d steps without kernel events: %.2f us/step (timed run: %.2f)" %
               (args.steps, (time.perf_counter() - t1) * 1e6 / args.steps, elapsed * 1e6 / args.steps),
               file=sys.stderr, flush=True)
-    sim.close()
+        sim.close()
 
     # Event counts of the timed steps: the same network, drive and steps again
     # with the device counters on (the step is deterministic; counting is kept

@@ -388,6 +388,86 @@ nc_status nc_read_kernel_timing(nc_sim* sim, uint32_t kernel, uint64_t* launches
 /* Message of the last failed call on this thread ("" if none). */
 const char* nc_last_error(void);
 
+/* ---------------------------------------------------------------------------
+ * Served form.  No counterpart in the reference, which runs one network per
+ * process (src/main.cpp:56-60): a native request loop for callers that
+ * submit many independent simulation jobs.  Jobs are built on builder
+ * threads (on the GPU with device_wiring), admitted into the server's
+ * simulations ("lanes", each with its own stream) while they have room,
+ * stepped together (one kernel pair per step covers every job of a lane;
+ * each job stops at its own step count), and retired at their step count.
+ * A job's result is exactly what it gives alone: the step of one network
+ * does not depend on the others.  With evict_after_s > 0, a job that has
+ * waited that long for room evicts the running job with the most steps
+ * left (nc_sim_save, then nc_sim_remove); the evicted job resumes later
+ * from its checkpoint (nc_sim_add, nc_sim_restore) with the same result.
+ * Every call is thread-safe; one internal thread drives the GPU.
+ * ------------------------------------------------------------------------ */
+typedef struct nc_server nc_server;
+
+typedef struct nc_server_spec {
+  int32_t device;
+  uint32_t lanes;          /* simulations stepped in turn (0 = 2) */
+  uint32_t chunk;          /* steps per lane turn (0 = 200) */
+  uint32_t builders;       /* host threads building job networks (0 = 4) */
+  uint32_t device_wiring;  /* 1: build job networks with nc_net_generate_device */
+  uint32_t max_nets;       /* jobs per lane (0 = 2048) */
+  uint64_t max_neurons;    /* capacity of each lane (0 = from hbm_fraction) */
+  uint64_t max_synapses;
+  float hbm_fraction;      /* with max_neurons = 0: the lanes together take this share
+                              of the device's free HBM, for networks of syn_per_neuron
+                              synapses per neuron (0 = 0.5) */
+  float syn_per_neuron;    /* (0 = 60) */
+  float evict_after_s;     /* 0 = never evict */
+  uint32_t overflow_lanes; /* extra lanes created while a job waits for room and free
+                              HBM (read when the job has waited evict_after_s, or 1 s
+                              without eviction) holds one; destroyed when empty (0 = none) */
+} nc_server_spec;
+
+typedef struct nc_job {
+  uint64_t id;             /* the caller's; returned in the result */
+  nc_wiring_spec wiring;   /* n_threads is ignored */
+  nc_drive_spec drive;
+  nc_params params;
+  uint32_t steps;          /* > 0 */
+} nc_job;
+
+typedef struct nc_job_result {
+  uint64_t id;
+  int32_t status;          /* NC_OK, or why the job did not run (NC_ERR_CAPACITY: larger than a lane) */
+  uint32_t evictions;      /* times the job was evicted and resumed */
+  uint32_t n_neurons;
+  uint32_t steps;
+  uint64_t n_synapses;
+  uint64_t spikes;         /* sum of the final firing counts */
+  uint64_t digest;         /* nc_digest of the final potentials, then weights (export
+                              order), then firing counts, each in neuron-id order */
+  double t_submit, t_built, t_admit, t_done;  /* seconds since nc_server_create */
+} nc_job_result;
+
+typedef struct nc_server_stats {
+  uint64_t submitted, built, admitted, finished, evictions, chunks;
+  uint32_t running, waiting, lanes;
+  double gpu_wait_s, admit_s, retire_s, idle_s;
+  double mean_running;     /* jobs per stepped chunk */
+} nc_server_stats;
+
+nc_status nc_server_create(const nc_server_spec* spec, nc_server** out);
+/* Queues a job (its arrival time is now). */
+nc_status nc_server_submit(nc_server* server, const nc_job* job);
+/* Moves up to cap finished jobs' results into out, waiting up to wait_s
+ * seconds for the first one. */
+nc_status nc_server_poll(nc_server* server, nc_job_result* out, uint32_t cap, uint32_t* n_out, double wait_s);
+nc_status nc_server_stats_get(nc_server* server, nc_server_stats* out);
+/* Stops the loop and frees everything; unfinished jobs are dropped.  Fails
+ * (and the server is still freed) if the loop had stopped on an error, whose
+ * message nc_last_error() then holds. */
+nc_status nc_server_destroy(nc_server* server);
+
+/* 64-bit FNV-1a of n bytes continued from h (start: 14695981039346656037),
+ * the digest of served results. */
+uint64_t nc_digest(const void* data, uint64_t n, uint64_t h);
+
 #ifdef __cplusplus
 }
 #endif

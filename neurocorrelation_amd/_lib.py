@@ -53,6 +53,37 @@ class LaunchCfg(C.Structure):
                 ("stage_tables", C.c_uint32), ("fanout_waves", C.c_uint32), ("persistent", C.c_uint32)]
 
 
+class ServerSpec(C.Structure):
+    _fields_ = [
+        ("device", C.c_int32), ("lanes", C.c_uint32), ("chunk", C.c_uint32), ("builders", C.c_uint32),
+        ("device_wiring", C.c_uint32), ("max_nets", C.c_uint32), ("max_neurons", C.c_uint64),
+        ("max_synapses", C.c_uint64), ("hbm_fraction", C.c_float), ("syn_per_neuron", C.c_float),
+        ("evict_after_s", C.c_float), ("overflow_lanes", C.c_uint32),
+    ]
+
+
+class JobSpec(C.Structure):
+    _fields_ = [("id", C.c_uint64), ("wiring", WiringSpec), ("drive", DriveSpec), ("params", Params),
+                ("steps", C.c_uint32)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [
+        ("id", C.c_uint64), ("status", C.c_int32), ("evictions", C.c_uint32), ("n_neurons", C.c_uint32),
+        ("steps", C.c_uint32), ("n_synapses", C.c_uint64), ("spikes", C.c_uint64), ("digest", C.c_uint64),
+        ("t_submit", C.c_double), ("t_built", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double),
+    ]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [
+        ("submitted", C.c_uint64), ("built", C.c_uint64), ("admitted", C.c_uint64), ("finished", C.c_uint64),
+        ("evictions", C.c_uint64), ("chunks", C.c_uint64), ("running", C.c_uint32), ("waiting", C.c_uint32),
+        ("lanes", C.c_uint32), ("gpu_wait_s", C.c_double), ("admit_s", C.c_double), ("retire_s", C.c_double),
+        ("idle_s", C.c_double), ("mean_running", C.c_double),
+    ]
+
+
 class SimSpec(C.Structure):
     _fields_ = [
         ("device", C.c_int32), ("max_neurons", C.c_uint64), ("max_synapses", C.c_uint64),
@@ -109,7 +140,15 @@ SIGNATURES = {
     "nc_set_kernel_timing": (C.c_int, [P, C.c_int, C.c_uint32]),
     "nc_read_kernel_timing": (C.c_int, [P, C.c_uint32, u64p, C.POINTER(C.c_double)]),
     "nc_last_error": (C.c_char_p, []),
+    "nc_server_create": (C.c_int, [C.POINTER(ServerSpec), C.POINTER(P)]),
+    "nc_server_submit": (C.c_int, [P, C.POINTER(JobSpec)]),
+    "nc_server_poll": (C.c_int, [P, C.POINTER(JobResult), C.c_uint32, u32p, C.c_double]),
+    "nc_server_stats_get": (C.c_int, [P, C.POINTER(ServerStats)]),
+    "nc_server_destroy": (C.c_int, [P]),
+    "nc_digest": (C.c_uint64, [C.c_void_p, C.c_uint64, C.c_uint64]),
 }
+
+FNV_OFFSET = 14695981039346656037
 
 _lib = None
 

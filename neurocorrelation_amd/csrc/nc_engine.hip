@@ -427,7 +427,7 @@ struct nc_sim {
   uint32_t *pend0 = nullptr, *pend1 = nullptr;
   uint2* pend_cnt = nullptr;
   uint32_t* ring = nullptr;        // arrival ring: NC_NB rows of max_mwords words
-  uint2* acache = nullptr;         // active-slot cache: NC_ACACHE rows of P entries
+  uint4* acache = nullptr;         // active-slot cache: one 16-byte entry per position
   uint32_t *id_pos = nullptr, *id_row = nullptr;  // per network, at its positions: neuron id -> local position, first synapse
   uint32_t* slice_moff = nullptr;  // per slice: first active-mask word
   uint32_t *slice_off = nullptr, *slice_net = nullptr;
@@ -553,7 +553,6 @@ nc_step_args base_args(nc_sim* sim) {
   a.ring = sim->ring;
   a.ring_words = (uint32_t)sim->max_mwords;
   a.acache = sim->acache;
-  a.acache_stride = (uint32_t)sim->max_pos;
   a.fired = sim->fired;
   a.fired_cnt = sim->fired_cnt;
   return a;
@@ -670,7 +669,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   const uint64_t slots = (n_syn + n_syn / 4 + P * NC_CHUNK + 255) / 256 * 256;  // padding allowance
   const uint64_t mwords = slots / 32 + 64 * Q;
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
-           + P * 8ull * NC_ACACHE                          // active-slot cache
+           + P * sizeof(uint4)                             // active-slot cache
            + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
            + (slots / 32 + 64 * Q) * 4ull * NC_NB + Q * 4     // arrival ring, slice mask offsets
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
@@ -757,7 +756,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->pend1, sim->max_slots * 4), "hipMalloc pend1");
     H(hipMalloc(&sim->pend_cnt, Q * sizeof(uint2)), "hipMalloc pend_cnt");
     H(hipMalloc(&sim->ring, sim->max_mwords * 4 * NC_NB), "hipMalloc ring");
-    H(hipMalloc(&sim->acache, P * 8ull * NC_ACACHE), "hipMalloc acache");
+    H(hipMalloc(&sim->acache, P * sizeof(uint4)), "hipMalloc acache");
     H(hipMalloc(&sim->slice_moff, Q * 4), "hipMalloc slice_moff");
     H(hipMalloc(&sim->id_pos, P * 4), "hipMalloc id_pos");
     H(hipMalloc(&sim->id_row, P * 4), "hipMalloc id_row");
@@ -789,7 +788,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMemsetAsync(sim->fired_cnt, 0, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->ring, 0, sim->max_mwords * 4 * NC_NB, sim->stream), "memset");
-    H(hipMemsetAsync(sim->acache, 0, P * 8ull * NC_ACACHE, sim->stream), "memset");
+    H(hipMemsetAsync(sim->acache, 0, P * sizeof(uint4), sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
     H(hipMemsetAsync(sim->d_chunk_off, 0, 1024, sim->stream), "memset");
@@ -1703,7 +1702,7 @@ extern "C" nc_status nc_sim_save(nc_sim* sim, int32_t net_id, void* buf, uint64_
   if (!buf) return NC_OK;
   if (cap < need) return fail(NC_ERR_RANGE, "checkpoint buffer too small");
   std::vector<nc_slot> hs;
-  st = read_slots(sim, *r, &hs);  // applies the deferred arrival STDP first
+  st = read_slots(sim, *r, &hs);
   if (st != NC_OK) return st;
   unsigned char* p = (unsigned char*)buf;
   nc_ckpt_header h{};
@@ -1787,7 +1786,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipMemcpy(hs.data(), sim->slots + r->slot0, r->nslots * sizeof(nc_slot), hipMemcpyDeviceToHost));
   std::vector<uint32_t> masks(r->nmwords, 0u);
   // the active-slot cache: each position's first NC_ACACHE active slots
-  std::vector<uint2> cache((size_t)NC_ACACHE * npos, make_uint2(0u, 0u));
+  std::vector<uint2> cache((size_t)NC_ACACHE * npos, make_uint2(0u, 0u));  // [position][entry]
   std::vector<uint8_t> ncache((size_t)npos, 0u);
   std::vector<std::vector<uint2>> pend(r->nq);
   std::vector<int32_t> edge_arr(r->S);
@@ -1812,7 +1811,7 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
         asum[pl] |= 1u << ((k >> 5) / G);
         uint32_t dbits;
         memcpy(&dbits, &x.depol, 4);
-        if (ncache[pl] < NC_ACACHE) cache[(size_t)ncache[pl]++ * npos + pl] = make_uint2(dbits, (uint32_t)arr);
+        if (ncache[pl] < NC_ACACHE) cache[(size_t)pl * NC_ACACHE + ncache[pl]++] = make_uint2(dbits, (uint32_t)arr);
       } else if (arr > s) {
         pend[q].push_back(make_uint2((uint32_t)r->slot0 + r->maps->slot_of_syn[syn], (uint32_t)arr));
       }
@@ -1842,9 +1841,8 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipMemcpyAsync(sim->asum + pos0, asum.data(), npos * 4, hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->amask + r->mask0, masks.data(), r->nmwords * 4, hipMemcpyHostToDevice, sq));
-  for (uint32_t e = 0; e < NC_ACACHE; ++e)
-    NC_HIP(hipMemcpyAsync(sim->acache + (size_t)e * sim->max_pos + pos0, cache.data() + (size_t)e * npos,
-                          npos * sizeof(uint2), hipMemcpyHostToDevice, sq));
+  static_assert(sizeof(uint4) == NC_ACACHE * sizeof(uint2), "one 16-byte cache entry per position");
+  NC_HIP(hipMemcpyAsync(sim->acache + pos0, cache.data(), npos * sizeof(uint4), hipMemcpyHostToDevice, sq));
   // each slice's spikes in flight into the pending list of their arrival
   // window, entries relative to the slice's first slot
   std::vector<uint2> cnt2(r->nq);
@@ -1942,7 +1940,7 @@ nc_status state_save(nc_sim* sim, StateCopy* c) {
       {sim->pend1, sim->max_slots * 4},
       {sim->pend_cnt, Q * sizeof(uint2)},
       {sim->ring, sim->max_mwords * 4 * NC_NB},
-      {sim->acache, P * 8ull * NC_ACACHE},
+      {sim->acache, P * sizeof(uint4)},
       {sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)},
       {sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4},
       {sim->forced_in, (P / 32) * 4},

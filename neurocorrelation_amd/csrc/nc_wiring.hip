@@ -36,16 +36,19 @@ namespace {
     if (e_ != hipSuccess) return fail(NC_ERR_HIP, hipGetErrorString(e_)); \
   } while (0)
 
-// Device buffers released together.
+// Device buffers released together, allocated and freed in the order of the
+// build's own stream (hipMallocAsync / hipFreeAsync), so that a build on one
+// host thread does not wait for, or hold up, work on other streams.
 struct Bufs {
+  hipStream_t st;
   std::vector<void*> p;
   ~Bufs() {
-    for (void* x : p) (void)hipFree(x);
+    for (void* x : p) (void)hipFreeAsync(x, st);
   }
   template <class T>
   hipError_t alloc(T** out, size_t n) {
     void* q = nullptr;
-    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+    hipError_t e = hipMallocAsync(&q, (n ? n : 1) * sizeof(T), st);
     if (e == hipSuccess) p.push_back(q);
     *out = (T*)q;
     return e;
@@ -229,9 +232,12 @@ extern "C" nc_status nc_net_generate_device(const nc_wiring_spec* spec, int32_t 
   NW_HIP(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
   struct StreamGuard {
     hipStream_t s;
-    ~StreamGuard() { (void)hipStreamDestroy(s); }
+    ~StreamGuard() {
+      (void)hipStreamSynchronize(s);  // the buffers' frees, queued on it
+      (void)hipStreamDestroy(s);
+    }
   } sg{st};
-  Bufs B;
+  Bufs B{st, {}};
   float *px, *py, *pz, *X, *Y, *Z, *xyz;
   uint32_t *cell_d, *cell, *cmem, *cstart, *cend;
   uint64_t *key, *key2, *ckey, *ckey2, *deg, *in_ptr;

@@ -14,15 +14,26 @@
 //       NeuCor(120) with two inputs for RUNS runs of 1 ms; the Neuron /
 //       Synapse mirror must match the snapshots, and a mirror setWeight must
 //       reach the engine (prints a summary; exit status 1 on a mismatch).
+//   facade_check standard SEED MS RUN_SPEED WALK_EVERY RATE SKIP
+//       the STANDARD preset of src/main.cpp:80-124 through the facade, as
+//       the reference harness's --standard mode runs it (runAll is true
+//       here whatever the caller sets): NeuCor(750) after srand(SEED), SKIP
+//       extra rand() draws, three inputs (randomRate, or RATE Hz if RATE >
+//       0) with the preset's random walk every WALK_EVERY-th frame, frames of
+//       RUN_SPEED ms until MS ms; fires counted by polling lastFire after
+//       every frame.  Prints "fires", "neurons_fired", "mean_w", "w_hist".
 //   facade_check grow RUNS
 //       the mirror after RUNS runs, then after a createNeuron/createSynapse
 //       and a run too short to step (the engine network rebuilt), then 50
 //       more runs.
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <exception>
+#include <vector>
 
 #include "NeuCor.h"
 
@@ -100,6 +111,7 @@ class NeuCor_Renderer {
   // What the renderer reads of each neuron and out-synapse
   // (src/NeuCor_Renderer.cpp:655-699, :840-866, :1349-1490):
   // "m FROM TO W AP_FIRETIME LASTARRIVAL" and "l ID LASTFIRE POTENTIAL" (hex).
+  static float last_fire(NeuCor& brain, std::size_t i) { return brain.neurons[i].lastFire; }
   static void dump(NeuCor& brain) {
     for (auto& neu : brain.neurons) {
       for (auto& syn : neu.outSynapses)
@@ -137,16 +149,76 @@ static int grow(long runs) {
   return 0;
 }
 
+static int standard(unsigned seed, double ms, float run_speed, int walk_every, float rate, int skip) {
+  srand(seed);
+  NeuCor brain(750);
+  brain.runSpeed = run_speed;
+  brain.runAll = false;  // the preset's setting; the engine steps every neuron every step
+  for (int k = 0; k < skip; ++k) (void)rand();
+  std::vector<float> inputs(3);
+  for (float& x : inputs) x = (float)rand() / (float)RAND_MAX * 75.0f;
+  if (rate > 0)
+    for (float& x : inputs) x = rate;
+  std::vector<float> radius(3, 0.8f);
+  std::vector<coord3> pos = {{cosf(0.0f) * 2.0f, sinf(0.0f) * 2.0f, 0.0f},
+                             {cosf(2.0944f) * 2.0f, sinf(2.0944f) * 2.0f, 0.0f},
+                             {cosf(4.1888f) * 2.0f, sinf(4.1888f) * 2.0f, 0.0f}};
+  brain.setInputRateArray(inputs.data(), 3, pos.data(), radius.data());
+  const std::size_t n = brain.getNeuronCount();
+  std::vector<float> last(n, NAN);
+  std::vector<char> ever(n, 0);
+  unsigned long long frames = 0, fires = 0;
+  while (brain.getTime() < ms) {
+    if (rate <= 0 && frames % (unsigned long long)walk_every == 0) {
+      for (float& x : inputs) {
+        x += ((float)rand() / (float)RAND_MAX - 0.5f) * 2.0f;
+        x = std::min(std::max(x, 0.0f), 75.0f);
+      }
+      inputs[1] = inputs[0];
+    }
+    brain.run();
+    ++frames;
+    for (std::size_t i = 0; i < n; ++i) {
+      const float lf = NeuCor_Renderer::last_fire(brain, i);
+      if (lf == lf && !(lf == last[i])) {
+        ++fires;
+        ever[i] = 1;
+        last[i] = lf;
+      }
+    }
+  }
+  unsigned long long hist[20] = {}, ns = 0, k = 0;
+  double sum_w = 0;
+  for (const auto& sn : brain.getSynapseSnapshots()) {
+    int b = (int)floorf((sn.weight + 1.0f) * 10.0f);
+    b = std::min(std::max(b, 0), 19);
+    ++hist[b];
+    sum_w += sn.weight;
+    ++ns;
+  }
+  for (char e : ever) k += e;
+  printf("time %.3f\nframes %llu\nfires %llu\nneurons_fired %llu\nsynapses %llu\nmean_w %.9g\nw_hist", brain.getTime(),
+         frames, fires, k, ns, ns ? sum_w / (double)ns : 0.0);
+  for (unsigned long long h : hist) printf(" %llu", h);
+  printf("\n");
+  return 0;
+}
+
 int main(int argc, char** argv) {
   try {
     if (argc == 4 && strcmp(argv[1], "wiring") == 0) return wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
     if (argc == 3 && strcmp(argv[1], "few") == 0) return few(atol(argv[2]));
     if (argc == 3 && strcmp(argv[1], "mirror") == 0) return NeuCor_Renderer::check(atol(argv[2]));
     if (argc == 3 && strcmp(argv[1], "grow") == 0) return grow(atol(argv[2]));
+    if (argc == 8 && strcmp(argv[1], "standard") == 0)
+      return standard((unsigned)strtoul(argv[2], 0, 10), atof(argv[3]), (float)atof(argv[4]),
+                      std::max(1, atoi(argv[5])), (float)atof(argv[6]), atoi(argv[7]));
   } catch (const std::exception& e) {
     fprintf(stderr, "facade_check: %s\n", e.what());
     return 1;
   }
-  fprintf(stderr, "usage: facade_check wiring N SEED | few RUNS | mirror RUNS | grow RUNS\n");
+  fprintf(stderr,
+          "usage: facade_check wiring N SEED | few RUNS | mirror RUNS | grow RUNS | "
+          "standard SEED MS RUN_SPEED WALK_EVERY RATE SKIP\n");
   return 2;
 }

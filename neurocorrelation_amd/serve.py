@@ -1,25 +1,33 @@
-"""The served form: a request loop over one device-resident simulation.
+"""The served form: simulation jobs over the engine's native request loop
+(nc_server_*, include/neucor_amd.h).
 
-Jobs (a network spec, a drive, a step count) arrive over time.  Each is
-built on a host thread as it arrives, admitted into the running simulation
-as soon as the simulation's device pools (sized from free HBM) have room for
-it, stepped together with every other admitted job (continuous batching: one
-step kernel pair covers all of them, each job stops at its own budget), and
-retired when its budget is spent, which frees its room for the jobs waiting.
+Jobs (a wiring spec, a drive, a step count) arrive over time.  The native
+server builds each one's network on builder threads (on the GPU with
+device_wiring), admits it into a running simulation ("lane") as soon as the
+lane has room, steps it together with every other admitted job (continuous
+batching: one step kernel pair covers all of a lane's jobs; each job stops at
+its own step count), and retires it at its step count, which frees its room.
 A job's result is exactly what it gives when run alone: the step of one
-network does not depend on the others in the simulation.
+network does not depend on the others.  Under overload, a job that has
+waited `evict_after` seconds evicts the running job with the most steps left
+(checkpoint, remove, resume later from the checkpoint, same result), and
+overflow lanes are created while the device's free HBM, read at that moment,
+holds one.
+
+This module holds the job specs of the served workload, the JSON wire format
+of jobs and results, and a thin wrapper over the native server.
 """
-import collections
-import concurrent.futures as cf
-import hashlib
+import ctypes as C
+import json
 import math
 import time
 
 import numpy as np
 
 from . import workloads
-from .engine import Simulation, footprint
-from ._lib import NCError
+from ._lib import (FNV_OFFSET, STATUS, JobResult, JobSpec, Params, ServerSpec, ServerStats, WiringSpec, check,
+                   lib)
+from .engine import Simulation, default_params
 
 
 def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps_range=(1_000, 50_000),
@@ -51,221 +59,187 @@ def served_specs(count, seed=0, n_range=(300, 100_000), k_range=(20, 100), steps
     return out
 
 
-def read_result(sim, nid, n, n_syn):
-    """A job's final state: potentials, weights, firing counts."""
-    return sim.potential(nid, n), sim.weights(nid, n_syn), sim.firings(nid, n)
+# ---------------------------------------------------------------------------
+# Wire format: one JSON object per job and per result (JSON lines).
+#   job    {"id": 7, "steps": 5000,
+#           "wiring": {"n_neurons", "side", "radius", "delay_min", "delay_max",
+#                      "w_min", "w_max", "inhib_frac", "seed"},
+#           "drive": {"n_inputs", "n_groups", "n_correlated", "period", "p0",
+#                     "amp_corr", "amp_ind", "seed"[, "bg_prob"]},
+#           "params": {"dt", "learning_rate", "pre_factor", "post_factor",
+#                      "pre_trace_decay", "post_trace_decay"}}   (optional)
+#   result {"id", "status" ("ok" or the error name), "spikes", "digest"
+#           (16 hex digits), "evictions", "n_neurons", "n_synapses", "steps",
+#           "t_submit", "t_built", "t_admit", "t_done"}
+# ---------------------------------------------------------------------------
+WIRING_KEYS = ("n_neurons", "side", "radius", "delay_min", "delay_max", "w_min", "w_max", "inhib_frac", "seed")
+DRIVE_KEYS = ("n_inputs", "n_groups", "n_correlated", "period", "p0", "amp_corr", "amp_ind", "seed")
+PARAM_KEYS = ("dt", "learning_rate", "pre_factor", "post_factor", "pre_trace_decay", "post_trace_decay")
+
+
+def job_to_json(job_id, spec):
+    out = {"id": int(job_id), "steps": int(spec["steps"]),
+           "wiring": {k: spec["wiring"][k] for k in WIRING_KEYS},
+           "drive": {k: spec["drive"][k] for k in DRIVE_KEYS + ("bg_prob",) if k in spec["drive"]}}
+    if "params" in spec:
+        out["params"] = {k: spec["params"][k] for k in PARAM_KEYS}
+    return json.dumps(out, sort_keys=True)
+
+
+def job_from_json(text):
+    """(job id, spec) of a job line; raises ValueError on a malformed job."""
+    d = json.loads(text)
+    try:
+        spec = {"steps": int(d["steps"]), "wiring": {k: d["wiring"][k] for k in WIRING_KEYS},
+                "drive": {k: d["drive"][k] for k in DRIVE_KEYS}}
+        if "bg_prob" in d["drive"]:
+            spec["drive"]["bg_prob"] = float(d["drive"]["bg_prob"])
+        if "params" in d:
+            spec["params"] = {k: float(d["params"][k]) for k in PARAM_KEYS}
+        job_id = int(d["id"])
+    except (KeyError, TypeError) as e:
+        raise ValueError("malformed job: %s" % e) from None
+    if spec["steps"] <= 0 or job_id < 0:
+        raise ValueError("malformed job: steps and id must be positive")
+    return job_id, spec
+
+
+def result_to_json(res):
+    return json.dumps(res, sort_keys=True)
+
+
+def job_struct(job_id, spec):
+    """The nc_job of a spec (params default to the reference's)."""
+    w = spec["wiring"]
+    ws = WiringSpec(int(w["n_neurons"]), 1, int(w["seed"]), float(w["side"]), float(w["radius"]),
+                    int(w["delay_min"]), int(w["delay_max"]), float(w["w_min"]), float(w["w_max"]),
+                    float(w.get("inhib_frac", 0.0)), 1)
+    ds = workloads.drive_spec(spec["drive"], seed=int(spec["drive"]["seed"]))
+    p = spec.get("params")
+    ps = default_params() if p is None else Params(*(float(p[k]) for k in PARAM_KEYS))
+    return JobSpec(int(job_id), ws, ds, ps, int(spec["steps"]))
+
+
+def digest_arrays(V, w, f):
+    """The served result digest (nc_digest) of final potentials, weights and
+    firing counts."""
+    h = FNV_OFFSET
+    for a in (np.ascontiguousarray(V, np.float32), np.ascontiguousarray(w, np.float32),
+              np.ascontiguousarray(f, np.uint32)):
+        h = lib().nc_digest(a.ctypes.data, a.nbytes, h)
+    return h
 
 
 def summarize(V, w, f):
-    """A job's result: its firing counts and a hash of its final potentials
-    and weights (bit patterns)."""
-    h = hashlib.sha256()
-    h.update(V.tobytes())
-    h.update(w.tobytes())
-    h.update(f.tobytes())
-    return {"spikes": int(f.sum()), "hash": h.hexdigest()}
+    return {"spikes": int(np.asarray(f, np.uint64).sum()), "digest": "%016x" % digest_arrays(V, w, f)}
 
 
-def digest(sim, nid, n, n_syn):
-    return summarize(*read_result(sim, nid, n, n_syn))
+def _result_dict(r):
+    out = {"id": int(r.id), "status": "ok" if r.status == 0 else STATUS.get(r.status, str(r.status)),
+           "evictions": int(r.evictions), "n_neurons": int(r.n_neurons), "n_synapses": int(r.n_synapses),
+           "steps": int(r.steps), "t_submit": r.t_submit, "t_built": r.t_built, "t_admit": r.t_admit,
+           "t_done": r.t_done}
+    if r.status == 0:
+        out["spikes"] = int(r.spikes)
+        out["digest"] = "%016x" % r.digest
+    return out
 
 
 class Job:
-    __slots__ = ("idx", "spec", "arrival", "built", "net", "drive", "params", "steps", "n", "n_syn", "nid",
-                 "t_built", "t_admit", "t_done", "result")
+    """A job of a served run: its index (the job id), spec and arrival time
+    (seconds from the start of the run); after the run, its build, admit and
+    completion times (same clock) and result."""
+    __slots__ = ("idx", "spec", "arrival", "t_built", "t_admit", "t_done", "result", "n", "n_syn", "steps",
+                 "evictions")
 
     def __init__(self, idx, spec, arrival):
         self.idx, self.spec, self.arrival = idx, spec, arrival
-        self.net = self.nid = self.t_built = self.t_admit = self.t_done = self.result = None
-
-
-def _finish(job, state, t0):
-    job.result = summarize(*state)
-    job.t_done = time.perf_counter() - t0
-    return job
-
-
-def _build(job):
-    net, drive, params, steps = workloads.build_spec(job.spec, n_threads=1)
-    net.prepare()  # the device image, off the serving thread
-    job.net, job.drive, job.params, job.steps = net, drive, params, steps
-    job.n, job.n_syn = net.n, net.n_syn
-    return job
-
-
-class _Lane:
-    """One simulation of the server and the jobs it runs."""
-
-    def __init__(self, sim):
-        self.sim = sim
-        self.running = {}  # net id -> job
+        self.t_built = self.t_admit = self.t_done = self.result = None
+        self.n = self.n_syn = self.evictions = 0
+        self.steps = int(spec["steps"])
 
 
 class Server:
-    """Simulations ("lanes", each with its own stream) sharing `max_neurons` /
-    `max_synapses` of device room and up to `max_nets` jobs; `chunk` steps
-    between admission and retirement checks.  With two lanes, the GPU runs
-    one lane's chunk while the host retires and admits the other's jobs."""
+    """The native server (nc_server_create).  max_neurons / max_synapses:
+    each lane's room (0: the lanes share `hbm_fraction` of the free HBM);
+    evict_after: seconds a job waits for room before it evicts (0: never);
+    overflow_lanes: extra lanes made from free HBM while jobs wait."""
 
-    def __init__(self, max_neurons, max_synapses, max_nets=4096, chunk=200, builders=8, device=0, max_ready=256,
-                 lanes=2, sim_factory=Simulation):
-        n, syn, nets = max_neurons // lanes, max_synapses // lanes, max(1, max_nets // lanes)
-        slots = syn + syn // 4 + (n + 64 * nets) * 4
-        self.lanes = [_Lane(sim_factory(n, syn, max_nets=nets, device=device, max_slots=slots))
-                      for _ in range(lanes)]
-        self.chunk = chunk
-        self.max_ready = max_ready  # jobs built or being built but not admitted (bounds host memory)
-        self.pool = cf.ThreadPoolExecutor(builders)
-        self.finisher = cf.ThreadPoolExecutor(2)  # results, apart from the builders' queue
+    def __init__(self, max_neurons=0, max_synapses=0, max_nets=2048, chunk=200, builders=4, device=0, lanes=2,
+                 device_wiring=False, evict_after=0.0, overflow_lanes=0, hbm_fraction=0.5, syn_per_neuron=60.0):
+        spec = ServerSpec(int(device), int(lanes), int(chunk), int(builders), int(bool(device_wiring)),
+                          int(max_nets), int(max_neurons), int(max_synapses), float(hbm_fraction),
+                          float(syn_per_neuron), float(evict_after), int(overflow_lanes))
+        h = C.c_void_p()
+        check(lib().nc_server_create(C.byref(spec), C.byref(h)))
+        self._h = h
+        self._created = time.perf_counter()
+        self.spec = spec
 
-    @property
-    def sim(self):
-        return self.lanes[0].sim
+    def submit(self, job_id, spec):
+        job = job_struct(job_id, spec)
+        check(lib().nc_server_submit(self._h, C.byref(job)))
 
-    @staticmethod
-    def capacity_for(free_bytes, fraction=0.5, syn_per_neuron=60):
-        """(max_neurons, max_synapses) whose footprint is about `fraction`
-        of `free_bytes`, for networks of about `syn_per_neuron` synapses."""
-        per_neuron = footprint(1_000_000, 1_000_000 * syn_per_neuron) / 1_000_000
-        n = int(free_bytes * fraction / per_neuron)
-        return n, n * syn_per_neuron
+    def poll(self, wait=0.0, cap=256):
+        """Finished jobs' results (dicts of the wire format), waiting up to
+        `wait` seconds for the first."""
+        buf = (JobResult * cap)()
+        n = C.c_uint32(0)
+        check(lib().nc_server_poll(self._h, buf, cap, C.byref(n), float(wait)))
+        return [_result_dict(buf[i]) for i in range(n.value)]
 
-    def run(self, jobs, t_limit=None, log=None, kernel_every=0):
-        """Serves `jobs` (each with an arrival time in seconds from now).
-        Returns (completed jobs with build/admit/done times and results,
-        summary) where summary holds the unfinished counts and where the
-        loop's wall time went: launching chunks, waiting for the GPU,
-        admission, retirement (result reads and removal) and idling.  With
-        kernel_every > 0 every kernel_every-th step's kernels are timed with
-        HIP events (mean microseconds in the summary)."""
+    def stats(self):
+        s = ServerStats()
+        check(lib().nc_server_stats_get(self._h, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in ServerStats._fields_}
+
+    def run(self, jobs, t_limit=None, log=None):
+        """Serves `jobs` (Job objects with arrival times in seconds from now):
+        submits each at its arrival time, collects results.  Returns
+        (completed jobs, summary) where summary holds the unfinished count and
+        the native loop's statistics."""
         t0 = time.perf_counter()
-        acc = collections.Counter()
-        if kernel_every:
-            for ln in self.lanes:
-                ln.sim.set_kernel_timing((1 << 1) | (1 << 3), kernel_every)
-        arrivals = sorted(jobs, key=lambda j: j.arrival)
-        nxt = 0
-        building = set()
-        waiting = collections.deque()
-        finishing = []  # results being summarised on the pool
+        off = t0 - self._created  # server clock at the start of the run
+        pending = sorted(jobs, key=lambda j: j.arrival)
+        by_id = {j.idx: j for j in jobs}
         done = []
-        in_flight = None  # the lane whose chunk the GPU is running
+        nxt = 0
         last_log = t0
-
-        def n_running():
-            return sum(len(ln.running) for ln in self.lanes)
-
-        while nxt < len(arrivals) or building or waiting or finishing or n_running():
+        while len(done) < len(jobs):
             now = time.perf_counter() - t0
             if t_limit is not None and now > t_limit:
                 break
-            served_any = False
-            for ln in self.lanes:
-                while (nxt < len(arrivals) and arrivals[nxt].arrival <= time.perf_counter() - t0
-                       and len(building) + len(waiting) < self.max_ready):
-                    building.add(self.pool.submit(_build, arrivals[nxt]))
-                    nxt += 1
-                for fut in [f for f in finishing if f.done()]:
-                    finishing.remove(fut)
-                    done.append(fut.result())
-                for fut in [f for f in building if f.done()]:
-                    building.discard(fut)
-                    job = fut.result()
-                    job.t_built = time.perf_counter() - t0
-                    waiting.append(job)
-                if ln is in_flight:
-                    if len(self.lanes) > 1:
-                        continue  # the other lanes are serviced while this one runs
-                    tw = time.perf_counter()
-                    ln.sim.synchronize()
-                    in_flight = None
-                    acc["gpu_wait_s"] += time.perf_counter() - tw
-                # retirement: this lane has no chunk in flight
-                tr = time.perf_counter()
-                for nid in list(ln.running):
-                    if ln.sim.progress(nid)[1] == 0:
-                        job = ln.running.pop(nid)
-                        state = read_result(ln.sim, nid, job.n, job.n_syn)
-                        ln.sim.remove(nid)
-                        finishing.append(self.finisher.submit(_finish, job, state, t0))
-                ta = time.perf_counter()
-                acc["retire_s"] += ta - tr
-                # admission in arrival order while this lane has room
-                while waiting:
-                    job = waiting[0]
-                    try:
-                        job.nid = ln.sim.add(job.net, job.params, job.drive, steps=job.steps)
-                    except NCError as e:
-                        # no room in the lane's pools (NC_ERR_CAPACITY) or for
-                        # its staging upload (NC_ERR_OOM): wait for room;
-                        # anything else is an error
-                        if e.code not in (4, 5):
-                            raise
-                        if ln.running:  # wait for room
-                            break
-                        # an empty lane has no room for it: no lane ever will
-                        waiting.popleft()
-                        job.result = {"error": "larger than the simulation's device room"}
-                        job.t_done = time.perf_counter() - t0
-                        job.net.close()
-                        job.net = None
-                        done.append(job)
-                        continue
-                    waiting.popleft()
-                    job.t_admit = time.perf_counter() - t0
-                    job.net.close()  # the simulation keeps what it needs
-                    job.net = None
-                    ln.running[job.nid] = job
-                tb = time.perf_counter()
-                acc["admit_s"] += tb - ta
-                # the GPU finishes the other lane's chunk, then takes this one
-                if in_flight is not None:
-                    in_flight.sim.synchronize()
-                    in_flight = None
-                tc = time.perf_counter()
-                acc["gpu_wait_s"] += tc - tb
-                if ln.running:
-                    ln.sim.step(self.chunk)
-                    in_flight = ln
-                    served_any = True
-                    acc["chunks"] += 1
-                    acc["running_sum"] += len(ln.running)
-                acc["launch_s"] += time.perf_counter() - tc
-            if not served_any and in_flight is None:
-                tb = time.perf_counter()
-                wait = 0.001
-                if nxt < len(arrivals) and not building:
-                    wait = max(0.0, min(0.01, arrivals[nxt].arrival - (time.perf_counter() - t0)))
-                time.sleep(wait)
-                acc["idle_s"] += time.perf_counter() - tb
+            while nxt < len(pending) and pending[nxt].arrival <= now:
+                self.submit(pending[nxt].idx, pending[nxt].spec)
+                nxt += 1
+            wait = 0.005
+            if nxt < len(pending):
+                wait = max(0.0, min(wait, pending[nxt].arrival - (time.perf_counter() - t0)))
+            for r in self.poll(wait=wait):
+                job = by_id[r["id"]]
+                job.t_built, job.t_admit, job.t_done = r["t_built"] - off, r["t_admit"] - off, r["t_done"] - off
+                job.n, job.n_syn, job.evictions = r["n_neurons"], r["n_synapses"], r["evictions"]
+                job.result = ({"spikes": r["spikes"], "digest": r["digest"]} if r["status"] == "ok"
+                              else {"error": r["status"]})
+                done.append(job)
             if log and time.perf_counter() - last_log > 10.0:
                 last_log = time.perf_counter()
-                log("served %d / %d jobs, %d running, %d waiting, %d arrived, %.1fs" %
-                    (len(done), len(arrivals), n_running(), len(waiting),
-                     sum(1 for j in arrivals if j.arrival <= last_log - t0), last_log - t0))
-        if in_flight is not None:
-            in_flight.sim.synchronize()
-        summary = {"unfinished": len(arrivals) - len(done), "running": n_running(), "waiting": len(waiting),
-                   "lanes": len(self.lanes),
-                   "loop": {k: (round(v, 3) if isinstance(v, float) else v) for k, v in acc.items()}}
-        if acc["chunks"]:
-            summary["loop"]["mean_running_per_chunk"] = acc["running_sum"] / acc["chunks"]
-        if kernel_every:
-            for kind, name in ((1, "k_neuron_us"), (3, "k_fanout_us")):
-                n = ms = 0
-                for ln in self.lanes:
-                    a, b = ln.sim.kernel_timing(kind)
-                    n, ms = n + a, ms + b
-                summary["loop"][name] = 1000.0 * ms / n if n else None
-            for ln in self.lanes:
-                ln.sim.set_kernel_timing(0)
-        return done, summary
+                st = self.stats()
+                log("served %d / %d jobs, %d running, %d waiting, %d evictions, %d lanes, %.1fs" %
+                    (len(done), len(jobs), st["running"], st["waiting"], st["evictions"], st["lanes"],
+                     last_log - t0))
+        return done, {"unfinished": len(jobs) - len(done), "loop": self.stats()}
 
     def close(self):
-        self.pool.shutdown(wait=True)
-        self.finisher.shutdown(wait=True)
-        for ln in self.lanes:
-            ln.sim.close()
+        if self._h:
+            h, self._h = self._h, None
+            check(lib().nc_server_destroy(h))
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def run_alone(spec):
@@ -273,6 +247,8 @@ def run_alone(spec):
     Returns (result, seconds including construction)."""
     t = time.perf_counter()
     net, drive, params, steps = workloads.build_spec(spec, n_threads=0)
+    if "params" in spec:
+        params = Params(*(float(spec["params"][k]) for k in PARAM_KEYS))
     sim = Simulation.for_network(net)
     nid = sim.add(net, params, drive)
     left = steps
@@ -281,7 +257,7 @@ def run_alone(spec):
         sim.step(k)
         left -= k
     sim.synchronize()
-    res = digest(sim, nid, net.n, net.n_syn)
+    res = summarize(sim.potential(nid, net.n), sim.weights(nid, net.n_syn), sim.firings(nid, net.n))
     el = time.perf_counter() - t
     sim.close()
     net.close()

@@ -31,7 +31,9 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <deque>
 #include <vector>
 
@@ -80,6 +82,142 @@ static int dump_wiring(int n, unsigned seed) {
   for (auto& nn : neu)
     for (auto& sy : nn.outSynapses)
       printf("s %zu %zu %a %a\n", nn.getID(), NeuCor_Renderer::target(sy), sy.getWeight(), NeuCor_Renderer::length(sy));
+  return 0;
+}
+
+// --standard SEED MS RUN_SPEED RUN_ALL WALK_EVERY RATE SKIP: the STANDARD
+// preset of src/main.cpp:80-124 headless: NeuCor(750) after srand(SEED);
+// three inputs on a circle of radius 2 (radius 0.8) at rates drawn as
+// randomRate does (:44-46); every WALK_EVERY-th frame the rates take the
+// preset's random walk (clamped to [0, 75], inputs[1] = inputs[0],
+// :100-106) before run(); frames of RUN_SPEED ms with runAll = RUN_ALL until
+// MS ms.  With RATE > 0 every input fires at RATE Hz throughout instead.
+// SKIP extra rand() draws after construction shift the C library's stream
+// (the background fires of src/NeuCor.cpp:604-607 draw from it).  Every
+// synapse's sign class is repaired first (setWeight(getWeight())): the
+// reference's Synapse copy leaves it unset when the out-synapse vectors grow
+// (src/NeuCor.cpp:487-524), and its value is then whatever the memory held.  Fires are counted by
+// polling lastFire after every frame (a neuron firing twice inside one frame
+// counts once).  Prints "frames", "fires", "neurons_fired", "mean_w" and
+// "w_hist" (20 bins over [-1, 1]).
+static int standard(unsigned seed, double ms, float run_speed, int run_all, int walk_every, float rate,
+                    int skip) {
+  g_libc_rand = true;
+  srand(seed);
+  NeuCor brain(750);
+  for (auto& n : NeuCor_Renderer::neurons(brain))
+    for (auto& sy : n.outSynapses) {
+      // the sign class, lost by vector growth (src/NeuCor.cpp:487-524)
+      sy.setWeight(sy.getWeight());
+    }
+  brain.runSpeed = run_speed;
+  brain.runAll = run_all != 0;
+  for (int k = 0; k < skip; ++k) (void)rand();
+  std::vector<float> inputs(3);
+  for (float& x : inputs) x = (float)rand() / (float)RAND_MAX * 75.0f;
+  if (rate > 0)
+    for (float& x : inputs) x = rate;
+  std::vector<float> radius(3, 0.8f);
+  std::vector<coord3> pos = {{cosf(0.0f) * 2.0f, sinf(0.0f) * 2.0f, 0.0f},
+                             {cosf(2.0944f) * 2.0f, sinf(2.0944f) * 2.0f, 0.0f},
+                             {cosf(4.1888f) * 2.0f, sinf(4.1888f) * 2.0f, 0.0f}};
+  brain.setInputRateArray(inputs.data(), 3, pos.data(), radius.data());
+  std::deque<Neuron>& neu = NeuCor_Renderer::neurons(brain);
+  std::vector<float> last(neu.size(), NAN);
+  std::vector<uint8_t> ever(neu.size(), 0);
+  uint64_t frames = 0, fires = 0;
+  while (brain.getTime() < ms) {
+    if (rate <= 0 && frames % (uint64_t)walk_every == 0) {
+      for (float& x : inputs) {
+        x += ((float)rand() / (float)RAND_MAX - 0.5f) * 2.0f;
+        x = std::min(std::max(x, 0.0f), 75.0f);
+      }
+      inputs[1] = inputs[0];
+    }
+    brain.run();
+    ++frames;
+    for (size_t i = 0; i < neu.size(); ++i) {
+      const float lf = neu[i].lastFire;
+      if (lf == lf && !(lf == last[i])) {
+        ++fires;
+        ever[i] = 1;
+        last[i] = lf;
+      }
+    }
+  }
+  uint64_t hist[20] = {};
+  double sum_w = 0;
+  uint64_t ns = 0;
+  for (auto& n : neu)
+    for (auto& sy : n.outSynapses) {
+      const float w = sy.getWeight();
+      int b = (int)std::floor((w + 1.0f) * 10.0f);
+      b = std::min(std::max(b, 0), 19);
+      ++hist[b];
+      sum_w += w;
+      ++ns;
+    }
+  uint64_t k = 0;
+  for (uint8_t e : ever) k += e;
+  printf("time %.3f\nframes %llu\nfires %llu\nneurons_fired %llu\nsynapses %llu\nmean_w %.9g\nw_hist",
+         brain.getTime(), (unsigned long long)frames, (unsigned long long)fires, (unsigned long long)k,
+         (unsigned long long)ns, ns ? sum_w / (double)ns : 0.0);
+  for (uint64_t h : hist) printf(" %llu", (unsigned long long)h);
+  printf("\n");
+  return 0;
+}
+
+// --gap RUN_ALL: neuron 2 receives synapses from neurons 0 and 1 (weight
+// 0.2, delay 4 ms); neuron 0 is fired at 1 ms and neuron 1 at 40 ms; frames
+// of 1 ms with runAll = RUN_ALL until 60 ms.  Prints "t V lastFire" of
+// neuron 2 after every frame.  With runAll = false the neuron runs only at
+// its events, and charge_insynapses (src/NeuCor.cpp:688-700) integrates a
+// still-latched synapse over the whole time since its last run.
+static int gap(int run_all) {
+  NeuCor brain(0);
+  brain.runSpeed = 1.0f;
+  brain.runAll = run_all != 0;
+  for (int i = 0; i < 3; ++i) brain.createNeuron(coord3{(float)i, 0.0f, 0.0f});
+  std::deque<Neuron>& neu = NeuCor_Renderer::neurons(brain);
+  brain.createSynapse(2, 0, 0.2f);
+  brain.createSynapse(2, 1, 0.2f);
+  for (auto& n : neu)
+    for (auto& sy : n.outSynapses) {
+      NeuCor_Renderer::set_length(sy, 2.0f);
+      sy.setWeight(sy.getWeight());
+    }
+  for (int f = 0; f < 60; ++f) {
+    brain.run();
+    if (f == 0) neu[0].fire();
+    if (f == 39) neu[1].fire();
+    printf("%g %a %a\n", brain.getTime(), neu[2].potential(), neu[2].lastFire);
+  }
+  return 0;
+}
+
+// --psp DELAY: neuron 1 receives one synapse from neuron 0 (weight 0.2,
+// delay DELAY ms = 2 * length); neuron 0 is fired at 1 ms; runAll, frames of
+// 1 ms until 12 ms.  Prints "t V" of neuron 1 after every frame.  An
+// arrival off the 1 ms grid runs the neuron at the arrival and again 2 ms
+// later (Synapse::run, Neuron::transfer, src/NeuCor.cpp:663-666, :718-726),
+// steps of fractional dt between the grid's.
+static int psp(float delay) {
+  NeuCor brain(0);
+  brain.runSpeed = 1.0f;
+  brain.runAll = true;
+  brain.createNeuron(coord3{0.0f, 0.0f, 0.0f});
+  brain.createNeuron(coord3{1.0f, 0.0f, 0.0f});
+  std::deque<Neuron>& neu = NeuCor_Renderer::neurons(brain);
+  brain.createSynapse(1, 0, 0.2f);
+  for (auto& sy : neu[0].outSynapses) {
+    NeuCor_Renderer::set_length(sy, delay / 2.0f);
+    sy.setWeight(sy.getWeight());
+  }
+  for (int f = 0; f < 12; ++f) {
+    brain.run();
+    if (f == 0) neu[0].fire();
+    printf("%g %a\n", brain.getTime(), neu[1].potential());
+  }
   return 0;
 }
 
@@ -132,6 +270,11 @@ int main(int argc, char** argv) {
     return 2;
   }
   if (argc == 4 && strcmp(argv[1], "--wiring") == 0) return dump_wiring(atoi(argv[2]), (unsigned)strtoul(argv[3], 0, 10));
+  if (argc == 3 && strcmp(argv[1], "--gap") == 0) return gap(atoi(argv[2]));
+  if (argc == 9 && strcmp(argv[1], "--standard") == 0)
+    return standard((unsigned)strtoul(argv[2], 0, 10), atof(argv[3]), (float)atof(argv[4]), atoi(argv[5]),
+                    std::max(1, atoi(argv[6])), (float)atof(argv[7]), atoi(argv[8]));
+  if (argc == 3 && strcmp(argv[1], "--psp") == 0) return psp((float)atof(argv[2]));
   int a = 1;
   uint32_t warmup = 0;  // --bench: untimed steps before the timed ones
   if (strcmp(argv[a], "--bench") == 0) {

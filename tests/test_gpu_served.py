@@ -1,6 +1,8 @@
-"""GPU: the served form.  Jobs arriving over time into one running
-simulation, admitted as room frees up and batched with whatever else is
-running, give exactly the result each gives alone on the plain path."""
+"""GPU: the served form (the native server, nc_server_*).  Jobs arriving
+over time into running simulations, admitted as room frees up and batched
+with whatever else is running, evicted and resumed under overload, built on
+the GPU or on the host, or admitted into an overflow lane, give exactly the
+result each gives alone on the plain path."""
 import sys
 
 import numpy as np
@@ -23,8 +25,10 @@ def test_served_jobs_equal_alone_under_queueing(lanes):
     rng = np.random.default_rng(5)
     arrivals = np.concatenate([[0.0] * 4, np.cumsum(rng.exponential(0.02, size=len(specs) - 4))])
     jobs = [SV.Job(i, sp, float(t)) for i, (sp, t) in enumerate(zip(specs, arrivals))]
-    # room for two jobs at a time: four arrive together, so some wait for room
-    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=2, chunk=64, builders=4, lanes=lanes)
+    # room for two jobs at a time over the lanes: four arrive together, so
+    # some wait for room
+    server = SV.Server(max_neurons=6000, max_synapses=150_000, max_nets=2 // lanes, chunk=64, builders=4,
+                       lanes=lanes)
     say("serving", len(jobs), "jobs on", lanes, "lane(s)")
     done, rest = server.run(jobs, t_limit=120)
     server.close()
@@ -45,16 +49,14 @@ def test_job_larger_than_the_room_is_refused():
     server.close()
     by = {j.idx: j for j in done}
     assert rest["unfinished"] == 0
-    assert "error" in by[1].result
-    assert "hash" in by[0].result and "hash" in by[2].result
+    assert by[1].result == {"error": "NC_ERR_CAPACITY"}
+    assert "digest" in by[0].result and "digest" in by[2].result
 
 
 def test_served_jobs_match_the_oracle(oracle_mod):
     """Two jobs of the served stream (seeded drive, 20% inhibitory weights,
     delays up to 32 steps), run in the served mix, equal the oracle's run of
     each job: firings, final potentials and weights."""
-    import hashlib
-
     specs = SV.served_specs(2, seed=21, n_range=(700, 1800), k_range=(40, 90), steps_range=(1500, 2500))
     jobs = [SV.Job(i, sp, 0.0) for i, sp in enumerate(specs)]
     server = SV.Server(max_neurons=8000, max_synapses=400_000, max_nets=4, chunk=100, builders=2, lanes=2)
@@ -73,9 +75,60 @@ def test_served_jobs_match_the_oracle(oracle_mod):
         say("oracle: job", job.idx, "n", net.n, "steps", steps)
         o.step(steps, cap=net.n * steps)
         st = o.state()
-        h = hashlib.sha256()
-        h.update(st["V"].astype(np.float32).tobytes())
-        h.update(st["w"].astype(np.float32).tobytes())
-        h.update(st["firings"].astype(np.uint32).tobytes())
-        assert job.result == {"spikes": int(st["firings"].sum()), "hash": h.hexdigest()}, job.idx
+        want = SV.summarize(st["V"], st["w"], st["firings"])
+        assert job.result == want, job.idx
         net.close()
+
+
+def _check_alone(done):
+    for job in sorted(done, key=lambda j: j.idx):
+        say("job", job.idx, "n", job.n, "steps", job.steps, "evictions", job.evictions)
+        res, _ = SV.run_alone(job.spec)
+        assert res == job.result, "job %d differs alone and in the mix" % job.idx
+
+
+def test_evicted_jobs_resume_with_the_same_result():
+    """One lane with room for two jobs; two long jobs take it, then short
+    jobs arrive.  A short job that waits 20 ms evicts the long job with the
+    most steps left (checkpointed, removed, resumed later from its
+    checkpoint)."""
+    long_ = SV.served_specs(2, seed=31, n_range=(1500, 2000), k_range=(30, 40), steps_range=(40_000, 50_000))
+    short = SV.served_specs(4, seed=32, n_range=(400, 900), k_range=(20, 30), steps_range=(100, 300))
+    jobs = [SV.Job(i, sp, 0.0) for i, sp in enumerate(long_)]
+    jobs += [SV.Job(2 + i, sp, 0.05 + 0.02 * i) for i, sp in enumerate(short)]
+    server = SV.Server(max_neurons=4500, max_synapses=120_000, max_nets=2, chunk=50, builders=2, lanes=1,
+                       evict_after=0.02)
+    say("serving with eviction")
+    done, rest = server.run(jobs, t_limit=120)
+    server.close()
+    assert rest["unfinished"] == 0 and len(done) == len(jobs)
+    assert rest["loop"]["evictions"] > 0
+    assert sum(j.evictions for j in done) == rest["loop"]["evictions"]
+    assert all("digest" in j.result for j in done)
+    _check_alone(done)
+
+
+def test_device_built_jobs_equal_host_built():
+    specs = SV.served_specs(4, seed=33, n_range=(300, 3000), k_range=(20, 60), steps_range=(150, 400))
+    jobs = [SV.Job(i, sp, 0.01 * i) for i, sp in enumerate(specs)]
+    server = SV.Server(max_neurons=20_000, max_synapses=800_000, max_nets=8, chunk=64, builders=2,
+                       device_wiring=True)
+    say("serving device-built jobs")
+    done, rest = server.run(jobs, t_limit=120)
+    server.close()
+    assert rest["unfinished"] == 0
+    _check_alone(done)
+
+
+def test_overflow_lane_takes_a_job_larger_than_a_lane():
+    small = SV.served_specs(2, seed=34, n_range=(400, 600), k_range=(20, 30), steps_range=(100, 200))
+    big = SV.served_specs(1, seed=35, n_range=(6000, 6000), k_range=(30, 30), steps_range=(200, 200))
+    jobs = [SV.Job(0, small[0], 0.0), SV.Job(1, big[0], 0.0), SV.Job(2, small[1], 0.01)]
+    server = SV.Server(max_neurons=3000, max_synapses=80_000, max_nets=4, chunk=50, builders=2,
+                       overflow_lanes=1)
+    say("serving with an overflow lane")
+    done, rest = server.run(jobs, t_limit=120)
+    server.close()
+    assert rest["unfinished"] == 0
+    assert all("digest" in j.result for j in done), [j.result for j in done]
+    _check_alone(done)
