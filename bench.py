@@ -32,39 +32,40 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
     the engine's event counters over the timed steps:
       20 B per neuron        in-degree/flags + input group 8, V read+write 8,
                              lastFire read 4
-      40 B per 64 neurons    active-slice record 16, pending counts 8,
-                             fired-lane record write 16
-       4 B per mask word     the arrival-ring word of the step (ceil(in-degree
-           of a neuron       / 32) per neuron), read
-       4 B per ring word     cleared after its arrivals joined the masks
+      48 B per 64 neurons    active-slice record 16, pending counts 8,
+                             fired-lane record write 16, arrival-bucket count
+                             read + reset 8
        4 B per mask word     read (words a summary bit flags) and 4 B per
                              word written back
        8 B per active slot   (depolarisation, arrival) from the active-slot
            in the cache      cache
       16 B per active slot   slot record, for active slots past the cache
            past the cache
-      16 B per arrival       slot record read (weight, depolarisation, arrival)
+       8 B per bucket entry  {slot, depolarisation} of an arrival
+      16 B per record read   at arrival (the target fired recently, or the
+                             spike came through a pending list)
        4 B per weight        written by the arrival's pre-side STDP
        8 B per cache entry   written (lanes whose active set changed)
       20 B per in-synapse    post-side STDP of a fired neuron: record read 16,
            of a fired neuron w write 4
       12 B per fired neuron  firings read+write 8, lastFire write 4
-       4 B per pending entry scanned (spikes whose delays exceed the arrival
-                             ring, in the list of the step's window)"""
+       4 B per pending entry scanned (spikes whose delays exceed NC_NB steps or
+                             found their bucket full, in the list of the step's
+                             window)"""
     d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree",
-                                               "pending_scanned", "mask_words_stepped", "mask_words_read",
-                                               "mask_words_written", "ring_words_cleared", "records_read",
-                                               "arrival_weight_writes", "cache_entries_written")}
+                                               "pending_scanned", "mask_words_read", "mask_words_written",
+                                               "bucket_entries", "records_read", "arrival_weight_writes",
+                                               "cache_entries_written", "arrival_records_read")}
     cached = d["active_slot_steps"] - d["records_read"]
-    b = (20.0 * n_neurons + 40.0 * n_neurons / 64 + 4.0 * (d["mask_words_stepped"] + d["ring_words_cleared"])
-         + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
-         + 8.0 * cached + 16.0 * d["records_read"] + 16.0 * d["arrivals"] + 4.0 * d["arrival_weight_writes"]
-         + 8.0 * d["cache_entries_written"] + 20.0 * d["fired_in_degree"] + 12.0 * d["fired"]
-         + 4.0 * d["pending_scanned"])
+    b = (20.0 * n_neurons + 48.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
+         + 8.0 * cached + 16.0 * d["records_read"] + 8.0 * d["bucket_entries"] + 16.0 * d["arrival_records_read"]
+         + 4.0 * d["arrival_weight_writes"] + 8.0 * d["cache_entries_written"] + 20.0 * d["fired_in_degree"]
+         + 12.0 * d["fired"] + 4.0 * d["pending_scanned"])
     return b, {"A": d["active_slot_steps"], "A_records": d["records_read"], "R": d["arrivals"],
+               "R_bucket": d["bucket_entries"], "R_records": d["arrival_records_read"],
                "R_w": d["arrival_weight_writes"], "cache_w": d["cache_entries_written"], "F": d["fired"],
                "F_in": d["fired_in_degree"], "pending": d["pending_scanned"], "mask_words": d["mask_words_read"],
-               "ring_words": d["mask_words_stepped"], "ring_cleared": d["ring_words_cleared"]}
+               "mask_words_written": d["mask_words_written"]}
 
 
 def fanout_kernel_bytes(c0, c1, steps):
@@ -74,12 +75,13 @@ def fanout_kernel_bytes(c0, c1, steps):
        8 B per fired neuron         out-row head {first edge, out-degree}
       16 B per out-edge of a        edge record {target slot, target slice |
            fired neuron             delay, latched arrival, target slice base}
-      36 B per accepted spike       target slot w, last arrival, arrival read
+      44 B per accepted spike       target slot w, last arrival, arrival read
                                     12, depol, last arrival, arrival write 12,
-                                    edge latch write 4, arrival-ring word OR
-                                    (read-modify-write) 8"""
+                                    edge latch write 4, bucket count
+                                    (read-modify-write) 8, bucket entry write 8
+                                    (or pending count 8 + entry 4)"""
     d = {k: (c1[k] - c0[k]) / steps for k in ("fired_records", "fired", "fired_out_degree", "accepted_spikes")}
-    b = 16.0 * d["fired_records"] + 8.0 * d["fired"] + 16.0 * d["fired_out_degree"] + 36.0 * d["accepted_spikes"]
+    b = 16.0 * d["fired_records"] + 8.0 * d["fired"] + 16.0 * d["fired_out_degree"] + 44.0 * d["accepted_spikes"]
     return b, {"records": d["fired_records"], "F": d["fired"], "F_out": d["fired_out_degree"],
                "accepted": d["accepted_spikes"]}
 

@@ -272,17 +272,18 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * [5] in-degree of fired, [6] out-degree of fired, [7] accepted spikes,
  * [8] pending-list entries scanned (spikes whose delays exceed the
  * arrival ring), [9] mask words of every stepped neuron, ceil(in-degree /
- * 32) each (the arrival-ring words a step reads), [10] active-mask words
+ * 32) each, [10] active-mask words
  * written back, [11] fired-slice records (64-neuron slices with at least one
  * fired neuron, summed over steps), [12] active-mask words read (those a
- * summary bit flags), [13] arrival-ring words cleared (words that held
- * arrivals), [14] slot records read by the charge (active slots past the
- * active-slot cache), [15] weights written at arrival (pre-side STDP that
- * changed the weight), [16] active-slot cache entries written.
+ * summary bit flags), [13] arrival-bucket entries taken, [14] slot records
+ * read by the charge (active slots past the active-slot cache), [15] weights
+ * written at arrival (pre-side STDP that changed the weight), [16]
+ * active-slot cache entries written, [17] slot records read at arrival (the
+ * target fired recently, or the spike came through a pending list).
  * [0], [1] and [9] are kept by the host; the others are device counters that
  * advance only while counting is on (nc_set_counting).
  * Writes the first n (at most NC_NUM_COUNTERS) into out. */
-#define NC_NUM_COUNTERS 17
+#define NC_NUM_COUNTERS 18
 nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
 
 /* Checkpoint of one network's dynamic state.  No counterpart in the

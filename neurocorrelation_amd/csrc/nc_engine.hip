@@ -426,7 +426,8 @@ struct nc_sim {
   nc_edge* edges = nullptr;
   uint32_t *pend0 = nullptr, *pend1 = nullptr;
   uint2* pend_cnt = nullptr;
-  uint32_t* ring = nullptr;        // arrival ring: NC_NB rows of max_mwords words
+  uint32_t* bcnt = nullptr;        // per slice: NC_NB arrival-bucket counts
+  uint2* bent = nullptr;           // per slice: NC_NB buckets of NC_BCAP entries
   uint4* acache = nullptr;         // active-slot cache: one 16-byte entry per position
   uint32_t *id_pos = nullptr, *id_row = nullptr;  // per network, at its positions: neuron id -> local position, first synapse
   uint32_t* slice_moff = nullptr;  // per slice: first active-mask word
@@ -550,8 +551,8 @@ nc_step_args base_args(nc_sim* sim) {
   a.pend0 = sim->pend0;
   a.pend1 = sim->pend1;
   a.pend_cnt = sim->pend_cnt;
-  a.ring = sim->ring;
-  a.ring_words = (uint32_t)sim->max_mwords;
+  a.bcnt = sim->bcnt;
+  a.bent = sim->bent;
   a.acache = sim->acache;
   a.fired = sim->fired;
   a.fired_cnt = sim->fired_cnt;
@@ -671,7 +672,7 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
            + P * sizeof(uint4)                             // active-slot cache
            + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
-           + (slots / 32 + 64 * Q) * 4ull * NC_NB + Q * 4     // arrival ring, slice mask offsets
+           + Q * NC_NB * (4ull + 8ull * NC_BCAP) + Q * 4     // arrival buckets, slice mask offsets
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
            + (n_syn + 64) * sizeof(nc_edge)                // out-edges
            + (Q + 2) * NC_NCTR * 8;                        // counters
@@ -737,10 +738,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
   sim->max_nets = spec->max_nets;
   sim->raster_cap = spec->raster_capacity;
   const uint64_t P = sim->max_pos, Q = P / 64;
-  // a whole number of 256-byte blocks, so that every arrival-ring row starts on one
-  sim->max_mwords = (sim->max_slots / 32 + 64 * Q + 63) / 64 * 64;
-  if (st == NC_OK && sim->max_mwords >= NC_MAX_MWORDS)
-    st = fail(NC_ERR_ARG, "capacity too large: more than 2^27 active-mask words (about 4 * 10^9 synapse slots)");
+  sim->max_mwords = sim->max_slots / 32 + 64 * Q;
   if (st == NC_OK) {
     H(hipMalloc(&sim->amask, sim->max_mwords * 4), "hipMalloc amask");
     H(hipMalloc(&sim->asum, P * 4), "hipMalloc asum");
@@ -755,7 +753,8 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->pend0, sim->max_slots * 4), "hipMalloc pend0");
     H(hipMalloc(&sim->pend1, sim->max_slots * 4), "hipMalloc pend1");
     H(hipMalloc(&sim->pend_cnt, Q * sizeof(uint2)), "hipMalloc pend_cnt");
-    H(hipMalloc(&sim->ring, sim->max_mwords * 4 * NC_NB), "hipMalloc ring");
+    H(hipMalloc(&sim->bcnt, Q * NC_NB * 4ull), "hipMalloc bcnt");
+    H(hipMalloc(&sim->bent, Q * NC_NB * NC_BCAP * sizeof(uint2)), "hipMalloc bent");
     H(hipMalloc(&sim->acache, P * sizeof(uint4)), "hipMalloc acache");
     H(hipMalloc(&sim->slice_moff, Q * 4), "hipMalloc slice_moff");
     H(hipMalloc(&sim->id_pos, P * 4), "hipMalloc id_pos");
@@ -787,7 +786,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMemsetAsync(sim->fired, 0, (Q + NC_FLISTS * 8ull) * sizeof(uint4), sim->stream), "memset");
     H(hipMemsetAsync(sim->fired_cnt, 0, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
-    H(hipMemsetAsync(sim->ring, 0, sim->max_mwords * 4 * NC_NB, sim->stream), "memset");
+    H(hipMemsetAsync(sim->bcnt, 0, Q * NC_NB * 4ull, sim->stream), "memset");
     H(hipMemsetAsync(sim->acache, 0, P * sizeof(uint4), sim->stream), "memset");
 
     H(hipMemsetAsync(sim->raster_cnt, 0, 64, sim->stream), "memset");
@@ -817,7 +816,7 @@ extern "C" void nc_sim_destroy(nc_sim* sim) {
   if (sim->stream) (void)hipStreamSynchronize(sim->stream);
   if (sim->graph) (void)hipGraphExecDestroy(sim->graph);
   void* ptrs[] = {sim->d_chunk_off, sim->amask, sim->asum, sim->slots, sim->V, sim->last_fire, sim->firings, sim->ndw,
-                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->ring, sim->acache, sim->slice_moff, sim->id_pos, sim->id_row, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
+                  sim->nid, sim->out_row, sim->edges, sim->pend0, sim->pend1, sim->pend_cnt, sim->bcnt, sim->bent, sim->acache, sim->slice_moff, sim->id_pos, sim->id_row, sim->slice_off, sim->slice_net, sim->active, sim->forced_in,
                   sim->forced_bg, sim->fired, sim->fired_cnt, sim->raster, sim->raster_cnt,
                   sim->ctr, sim->d_nets, sim->t_pre, sim->t_post, sim->w_ap, sim->d_groups.p,
                   sim->d_thr.p, sim->d_ev_pos.p, sim->d_ev_kind.p, sim->d_gfire.p, sim->d_wbuf.p, sim->d_stage.p};
@@ -1068,8 +1067,7 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   if (S) {
     NC_HIP(hipMemcpyAsync(stg + o_jq, I.edge_jq.data(), S * sizeof(uint2), hipMemcpyHostToDevice, st));
     hipLaunchKernelGGL(k_place_edges, blocks(S), dim3(256), 0, st, sim->edges + o0, (const uint2*)(stg + o_jq), S,
-                       (uint32_t)sl0, (uint32_t)q0, (const uint32_t*)sim->slice_off,
-                       (const uint32_t*)sim->slice_moff);
+                       (uint32_t)sl0, (uint32_t)q0, (const uint32_t*)sim->slice_off);
     NC_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_place_neurons, blocks(npos), dim3(256), 0, st, sim->V + pos0, sim->last_fire + pos0,
@@ -1088,12 +1086,8 @@ extern "C" nc_status nc_sim_add(nc_sim* sim, const nc_net* net, const nc_params*
   }
   NC_HIP(hipMemcpyAsync(sim->out_row + pos0, orow.data(), npos * sizeof(uint2), hipMemcpyHostToDevice, st));
   NC_HIP(hipMemsetAsync(sim->pend_cnt + q0, 0, I.nq * sizeof(uint2), st));
-  if (I.nmwords) {
-    // no spike of a network that held these words before may arrive here
-    hipLaunchKernelGGL(k_clear_ring, blocks(NC_NB * I.nmwords), dim3(256), 0, st, sim->ring,
-                       (uint32_t)sim->max_mwords, (uint32_t)mk0, (uint32_t)I.nmwords);
-    NC_HIP(hipGetLastError());
-  }
+  // no spike of a network that held these slices before may arrive here
+  NC_HIP(hipMemsetAsync(sim->bcnt + (size_t)q0 * NC_NB, 0, (size_t)I.nq * NC_NB * 4, st));
   NC_HIP(hipMemcpyAsync(sim->slice_net + q0, snet.data(), I.nq * 4, hipMemcpyHostToDevice, st));
   NC_HIP(hipMemcpyAsync(sim->d_nets + id, &nd, sizeof(nd), hipMemcpyHostToDevice, st));
   NC_HIP(hipStreamSynchronize(st));
@@ -1939,7 +1933,8 @@ nc_status state_save(nc_sim* sim, StateCopy* c) {
       {sim->pend0, sim->max_slots * 4},
       {sim->pend1, sim->max_slots * 4},
       {sim->pend_cnt, Q * sizeof(uint2)},
-      {sim->ring, sim->max_mwords * 4 * NC_NB},
+      {sim->bcnt, Q * NC_NB * 4ull},
+      {sim->bent, Q * NC_NB * NC_BCAP * sizeof(uint2)},
       {sim->acache, P * sizeof(uint4)},
       {sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)},
       {sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4},

@@ -3,25 +3,25 @@
 // One step s of every active network is, in order:
 //   k_drive   host-scheduled forced fires -> bitmasks       (only when present)
 //   k_neuron  one lane per neuron (one wave per 64-neuron slice):
-//             the slice's spikes arriving at s from its pending list join
-//             the arrival ring's row of s; charge_insynapses over the
-//             neuron's active slots (depolarisation and arrival of the first
-//             NC_ACACHE from the lane's active-slot cache, the rest from
-//             their slot records) with the latch releases, charge_passive,
-//             threshold / forced fire, AP waveform; then the wave applies
-//             post-side STDP to the incoming synapses of each of its fired
-//             lanes, and each lane takes up its arrivals at s from the ring
-//             (pre-side STDP, active bits, cache)
+//             charge_insynapses over the neuron's active slots
+//             (depolarisation and arrival of the first NC_ACACHE from the
+//             lane's active-slot cache, the rest from their slot records)
+//             with the latch releases, charge_passive, threshold / forced
+//             fire, AP waveform; then the wave applies post-side STDP to the
+//             incoming synapses of each of its fired lanes, and takes up the
+//             slice's arrivals at s -- its bucket of step s, then its pending
+//             list -- each by the lane that owns the slot (pre-side STDP,
+//             active bits, cache)
 //   k_fanout  waves sharing the step's records of slices with fired lanes
 //             (64 lists, one record per wave below a cap): Synapse::fire on
 //             the fired lanes' outgoing synapses (source-order latch copy,
-//             out->in slot map), marking accepted spikes in the ring row of
-//             their arrival step, or, past the ring, appending them to the
-//             target slice's pending list of their arrival's window
+//             out->in slot map), appending accepted spikes to the target
+//             slice's bucket of their arrival step, or, past NC_NB steps or
+//             a full bucket, to its pending list of their arrival's window
 // The kernel boundary orders the phases.  Every synapse slot has one writer
 // per phase (its target's wave in k_neuron, its source's wave in k_fanout);
-// the atomics are the ring bits, the appends to the pending lists (per
-// target slice) and to the fired lists (per list), and the firing count.
+// the atomics are the appends to the buckets and pending lists (per target
+// slice) and to the fired lists (per list), and the firing count.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -39,13 +39,15 @@
 // (slot - first slot of the slice) << 8 | (arrival & 255): a list scanned at
 // step s holds no arrival before s nor past s + 255 (delays are 1..255).
 #define NC_PEND_LOG_W 4
-// Spikes with delays up to NC_NB steps go to the arrival ring instead: NC_NB
-// rows shaped like the active masks (one bit per slot), row a % NC_NB for the
-// arrivals at step a.  Fan-out sets the slot's bit with an atomic OR that
-// returns nothing; the step of arrival a reads its lanes' words of row
-// a % NC_NB (each lane its own words), ORs them into the active masks and
-// clears them.  Longer delays take the pending lists.
+// Spikes with delays up to NC_NB steps go to an arrival bucket instead: per
+// slice, one bucket per arrival step modulo NC_NB, of up to NC_BCAP entries
+// (slot - first slot of the slice, latched depolarisation).  Fan-out takes
+// the place with an atomic on the bucket's count; the step of arrival a reads
+// bucket a % NC_NB of each slice -- exactly its arrivals, with what they
+// contribute -- and empties it.  A spike that finds its bucket full, or
+// whose delay exceeds NC_NB, goes to the pending list of its window.
 #define NC_NB 64u
+#define NC_BCAP 64u
 // Slices with fired lanes go to one of NC_FLISTS lists (by wave index), so
 // that the appends of a step spread over that many counters.
 #define NC_FLISTS 64
@@ -55,7 +57,7 @@
 #define NC_CHUNK_LOG 3
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
-#define NC_NCTR 17  // counter columns (device-side: 2-8, 10-16)
+#define NC_NCTR 18  // counter columns (device-side: 2-8, 10-17)
 // Each lane keeps (depolarisation, arrival) of its first NC_ACACHE active
 // slots, in slot order, in a per-position cache entry (16 bytes: the pairs
 // of position p at acache[p]): the charge reads those with one coalesced
@@ -97,9 +99,8 @@ struct nc_edge {
   uint32_t tq_d;   // target slice << 8 | delay in steps
   int32_t arr;     // arrival step of the last accepted spike (NC_NEVER: none);
                    // the latch holds until arrival + release window
-  uint32_t mwb;    // the slot's active-mask word << 5 | its bit (its place in an arrival-ring row)
+  uint32_t rel;    // the slot's place in its slice (slot - the slice's first slot)
 };
-#define NC_MAX_MWORDS (1u << 27)  // mask words of a simulation (mwb holds 27 bits of word index)
 
 struct nc_group {
   uint32_t tbl_off;  // first threshold entry
@@ -157,8 +158,8 @@ struct nc_step_args {
   uint32_t* pend0;         // pending list 0 of every slice, at its slot range
   uint32_t* pend1;         // pending list 1
   uint2* pend_cnt;         // per slice: entries in pending list 0, in list 1
-  uint32_t* ring;          // NC_NB rows of ring_words words (the arrival ring)
-  uint32_t ring_words;     // words per ring row (every mask word of the simulation)
+  uint32_t* bcnt;          // per slice: NC_NB arrival-bucket counts (beyond NC_BCAP: the rest overflowed)
+  uint2* bent;             // per slice: NC_NB buckets of NC_BCAP entries {slot - first slot, depol bits}
   uint4* acache;           // per position: (depol bits, arrival) of its first two active slots
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
@@ -298,33 +299,16 @@ __global__ __launch_bounds__(256) void k_gather_w(const nc_slot* slots, uint64_t
 // the target slice's first slot and first mask word (already in place), no
 // spike latched.
 __global__ __launch_bounds__(256) void k_place_edges(nc_edge* e, const uint2* jq, uint64_t n, uint32_t slot0,
-                                                     uint32_t q0, const uint32_t* slice_off,
-                                                     const uint32_t* slice_moff) {
+                                                     uint32_t q0, const uint32_t* slice_off) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
     const uint2 x = jq[i];
     const uint32_t tq_d = x.y + (q0 << 8);
     const uint32_t j = x.x + slot0, tq = tq_d >> 8;
-    // slot j = first slot + (k >> 3) * 512 + lane * 8 + (k & 7) (chunk-major)
-    const uint32_t rel = j - slice_off[tq];
-    const uint32_t lane = (rel >> NC_CHUNK_LOG) & 63u;
-    const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
-    const uint32_t mword = slice_moff[tq] + (k >> 5) * 64u + lane;
-    e[i] = nc_edge{j, tq_d, NC_NEVER, (mword << 5) | (k & 31u)};
+    e[i] = nc_edge{j, tq_d, NC_NEVER, j - slice_off[tq]};
   }
 }
 
-// Clears the arrival-ring words of a mask-word range in every row (a network
-// placed where another one's spikes may still have been in flight).
-__global__ __launch_bounds__(256) void k_clear_ring(uint32_t* ring, uint32_t ring_words, uint32_t w0, uint32_t nw) {
-  const uint64_t n = (uint64_t)NC_NB * nw;
-  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
-  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    ring[(i / nw) * ring_words + w0 + i % nw] = 0u;
-}
-
-// A network's slot records from their uploaded weights and sign bits (one bit
-// per slot): no spike latched or arrived yet.
 __global__ __launch_bounds__(256) void k_place_slots(nc_slot* s, const float* w, const uint32_t* inh, uint64_t n) {
   const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
@@ -399,26 +383,33 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   const nc_netdesc nd = a.nets[rec.net];
   const bool live = wave_in && ci < nd.remaining;
   const int32_t s = nd.base_step + ci + 1;
-  // The pending list of s's arrival window (spikes whose delays exceed the
-  // arrival ring).
+  // The arrival bucket of step s (spikes with delays up to NC_NB that found
+  // room in it) and the pending list of s's arrival window (the others).
+  const size_t bi = (size_t)q * NC_NB + ((uint32_t)s & (NC_NB - 1u));
+  uint32_t b_n = a.bcnt[bi];
   const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
   uint32_t pend_n = plist ? pc.y : pc.x;
   if (!live) {
     nw = make_uint2(0u, 0u);
     pend_n = 0u;
     sm = 0u;
+    b_n = 0u;
   }
   uint32_t* const pl = (plist ? a.pend1 : a.pend0) + rec.slot_off;
   const uint32_t dw = nw.x;
   const bool valid = (dw & NC_DEG_VALID) != 0;
   const uint32_t deg = dw & NC_DEG_MASK;
   const float lr = nd.lr, pre_f = nd.pre_f, post_f = nd.post_f;
-  uint32_t n_active = 0, n_arr = 0, n_mwb = 0, n_mwr = 0, n_ring = 0, n_rec = 0, n_wset = 0, n_cw = 0;
+  uint32_t n_active = 0, n_arr = 0, n_mwb = 0, n_mwr = 0, n_rec = 0, n_arec = 0, n_wset = 0, n_cw = 0;
   const uint32_t nmw = (deg + 31) >> 5;
   const uint32_t G = nmw <= 32 ? 1u : (nmw + 31) >> 5;  // mask words per summary bit
   uint32_t* const mrow = a.amask + rec.mask_off + lane;
-  uint32_t* const ring_s = a.ring + (size_t)((uint32_t)s & (NC_NB - 1u)) * a.ring_words + rec.mask_off;
-  uint32_t* const rrow = ring_s + lane;
+  // The bucket's entries, fetched now and taken up after the fire; the
+  // bucket is emptied for its next use (step s + NC_NB).
+  const uint32_t b_take = b_n < NC_BCAP ? b_n : NC_BCAP;
+  uint2 be = make_uint2(0u, 0u);
+  if (lane < b_take) be = a.bent[bi * NC_BCAP + lane];
+  if (b_n && lane == 0) a.bcnt[bi] = 0u;
   // The lane's cache of (depolarisation, arrival) of its first NC_ACACHE
   // active slots in slot order, read when the summary says it has any.
   uint2 ac[NC_ACACHE] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
@@ -426,35 +417,6 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     const uint4 c = a.acache[pos];
     ac[0] = make_uint2(c.x, c.y);
     ac[1] = make_uint2(c.z, c.w);
-  }
-
-  // Spikes of this slice arriving at s from the pending list join the ring
-  // row of s (any lane's words: an atomic OR at workgroup scope), where each
-  // lane takes them up with the ring's own arrivals after the fire; the list
-  // is compacted in place (write index <= read index).
-  if (pend_n) {
-    uint32_t kept = 0;
-    for (uint32_t t0 = 0; t0 < pend_n; t0 += 64) {
-      const uint32_t t = t0 + lane;
-      const uint32_t e = t < pend_n ? pl[t] : 0u;
-      const bool arrive_now = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
-      if (arrive_now) {
-        const uint32_t rel = e >> 8;
-        const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
-        const uint32_t k = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
-        __hip_atomic_fetch_or(ring_s + (k >> 5) * 64 + tl, 1u << (k & 31), __ATOMIC_RELAXED,
-                              __HIP_MEMORY_SCOPE_WORKGROUP);
-      }
-      const bool keep = t < pend_n && !arrive_now;
-      const unsigned long long km = __ballot(keep);
-      const uint32_t to = kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull));
-      if (keep && to != t) pl[to] = e;
-      kept += (uint32_t)__builtin_popcountll(km);
-    }
-    if (lane == 0) reinterpret_cast<uint32_t*>(&a.pend_cnt[q])[plist] = kept;
-    // the lanes read their ring words below with plain loads: after the ORs
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
   }
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
@@ -633,68 +595,90 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
 
-  // Spikes arriving at s: the lane's words of the ring row of s.  Each one
-  // (Synapse::run, :718-726) takes its pre-side STDP now -- synapse trace
-  // powf(decay, 0) == 1 -> 0, target trace of lastFire after this step; its
-  // record already holds the arrival as lastSpikeArrival -- joins the masks
-  // (contributing from s+1) and the next cache.
+  // Spikes arriving at s (Synapse::run, :718-726): the bucket's entries,
+  // which carry the latched depolarisation, then the pending list's
+  // arrivals, whose depolarisation is read from the slot record.  Each one
+  // takes its pre-side STDP now -- synapse trace powf(decay, 0) == 1 -> 0,
+  // target trace of lastFire after this step; the record already holds the
+  // arrival as lastSpikeArrival, and is read only when the target trace is
+  // not 0 -- and joins the masks (contributing from s+1) and the next cache.
+  // An entry belongs to the lane that owns its slot: six ballots on the bits
+  // of the target lane give each lane the set of entries that are its own.
   const float tr_t = nc_lookup(tpost, a.len_post, s - lf);
-  for (uint32_t w0 = 0; w0 < nmw; w0 += 4) {
-    uint32_t rw[4];
+  uint32_t arrw = 0;  // which of mask words 0-3 took arrivals
+  // batch 0: the bucket's entries; batch b > 0: pending entries 64 (b - 1) ..
+  const uint32_t n_batches = 1u + (pend_n + 63u) / 64u;
+  uint32_t kept = 0;  // pending entries kept (compacted in place: write index <= read index)
+  for (uint32_t bt = (b_take ? 0u : 1u); bt < n_batches; ++bt) {
+    const bool known = bt == 0u;  // bucket entries carry the depolarisation
+    uint32_t rel = be.x, dbits = be.y, e = 0u, t = 0u;
+    bool ev = lane < b_take;
+    if (!known) {
+      t = (bt - 1u) * 64u + lane;
+      e = t < pend_n ? pl[t] : 0u;
+      ev = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
+      rel = e >> 8;
+      dbits = 0u;
+    }
+    // entry (slot - first slot of the slice) -> the lane that owns the slot
+    const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
+    const uint32_t kk = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
+    unsigned long long mine = __ballot(ev);
 #pragma unroll
-    for (int u = 0; u < 4; ++u) rw[u] = w0 + u < nmw ? rrow[(w0 + u) * 64] : 0u;
-    unsigned long long lo = ((unsigned long long)rw[1] << 32) | rw[0];
-    unsigned long long hi = ((unsigned long long)rw[3] << 32) | rw[2];
-    while (lo | hi) {
-      uint32_t pp[NC_RB], jj[NC_RB];
-      uint32_t cnt = 0;
+    for (int bb = 0; bb < 6; ++bb) {
+      const unsigned long long m = __ballot(ev && ((tl >> bb) & 1u));
+      mine &= ((lane >> bb) & 1u) ? m : ~m;
+    }
+    while (__ballot(mine != 0ull)) {
+      const uint32_t src = mine ? (uint32_t)__builtin_ctzll(mine) : lane;
+      const uint32_t k = (uint32_t)__shfl((int)kk, (int)src);
+      uint32_t dk = (uint32_t)__shfl((int)dbits, (int)src);
+      if (mine) {
+        mine &= mine - 1;
+        const uint32_t w = k >> 5, bit = 1u << (k & 31u);
+        if (w < 4u) {
 #pragma unroll
-      for (int e = 0; e < NC_RB; ++e) {
-        uint32_t p = 0;
-        if (lo) {
-          p = (uint32_t)__builtin_ctzll(lo);
-          lo &= lo - 1;
-          cnt = e + 1;
-        } else if (hi) {
-          p = 64u + (uint32_t)__builtin_ctzll(hi);
-          hi &= hi - 1;
-          cnt = e + 1;
+          for (int u = 0; u < 4; ++u)
+            if (w == (uint32_t)u) kw[u] |= bit;
+          arrw |= 1u << w;
+        } else {
+          mrow[w * 64] |= bit;
+          ++n_mwb;
         }
-        pp[e] = p;
-        jj[e] = nc_slot_of(rec.slot_off, lane, 32u * w0 + p);
-      }
-      nc_slot r[NC_RB];
-#pragma unroll
-      for (int e = 0; e < NC_RB; ++e)
-        if ((uint32_t)e < cnt) r[e] = a.slots[jj[e]];
-#pragma unroll
-      for (int e = 0; e < NC_RB; ++e) {
-        if ((uint32_t)e >= cnt) break;
-        if (tr_t != 0.0f) {
-          const float wn = nc_stdp(r[e].w, (uint32_t)r[e].arr_inh & 1u, 0.0f, tr_t, pre_f, post_f, lr);
-          if (__float_as_uint(wn) != __float_as_uint(r[e].w)) {
-            a.slots[jj[e]].w = wn;
-            ++n_wset;
+        sm_keep |= 1u << (G == 1u ? w : w / G);
+        if (tr_t != 0.0f || !known) {
+          const uint32_t j = nc_slot_of(rec.slot_off, lane, k);
+          const nc_slot r = a.slots[j];
+          ++n_arec;
+          if (!known) dk = __float_as_uint(r.depol);
+          if (tr_t != 0.0f) {
+            const float wn = nc_stdp(r.w, (uint32_t)r.arr_inh & 1u, 0.0f, tr_t, pre_f, post_f, lr);
+            if (__float_as_uint(wn) != __float_as_uint(r.w)) {
+              a.slots[j].w = wn;
+              ++n_wset;
+            }
           }
         }
-        nc_push(nk, nv, ncnt, 32u * w0 + pp[e], make_uint2(__float_as_uint(r[e].depol), (uint32_t)s));
+        nc_push(nk, nv, ncnt, k, make_uint2(dk, (uint32_t)s));
         cache_dirty = true;
+        ++n_arr;
       }
-      n_arr += cnt;
     }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const uint32_t w = w0 + u;
-      if (w >= nmw || !rw[u]) continue;
-      const bool flagged = (sm >> (G == 1u ? w : w / G)) & 1u;
-      const uint32_t base = w0 == 0 ? kw[u] : (flagged ? mrow[w * 64] : 0u);
-      mrow[w * 64] = base | rw[u];
-      rrow[w * 64] = 0u;
-      ++n_mwb;
-      ++n_ring;
-      sm_keep |= 1u << (G == 1u ? w : w / G);
+    if (!known) {
+      const bool keep = t < pend_n && !ev;
+      const unsigned long long km = __ballot(keep);
+      const uint32_t to = kept + (uint32_t)__builtin_popcountll(km & ((1ull << lane) - 1ull));
+      if (keep && to != t) pl[to] = e;
+      kept += (uint32_t)__builtin_popcountll(km);
     }
   }
+  if (pend_n && lane == 0) reinterpret_cast<uint32_t*>(&a.pend_cnt[q])[plist] = kept;
+#pragma unroll
+  for (int u = 0; u < 4; ++u)
+    if ((arrw >> u) & 1u) {
+      mrow[u * 64] = kw[u];
+      ++n_mwb;
+    }
   if (cache_dirty) {
     // one 16-byte store; entries past ncnt are never read (the lane has
     // ncnt active slots)
@@ -732,10 +716,11 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     nc_count32(a.ctr, blk, 10, n_mwb);
     if (lane == 0 && pend_n) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 8], (unsigned long long)pend_n);
     nc_count32(a.ctr, blk, 12, n_mwr);
-    nc_count32(a.ctr, blk, 13, n_ring);
+    if (lane == 0 && b_take) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 13], (unsigned long long)b_take);
     nc_count32(a.ctr, blk, 14, n_rec);
     nc_count32(a.ctr, blk, 15, n_wset);
     nc_count32(a.ctr, blk, 16, n_cw);
+    nc_count32(a.ctr, blk, 17, n_arec);
   }
 }
 
@@ -770,8 +755,8 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
 // fired neuron: each wave takes every (nwaves / 64)-th record of one of the
 // step's 64 fired-slice lists, 64 per load, and the slices one by one; one
 // writer per synapse (its source), latch permitting.  Accepted spikes latch
-// depolarisation and arrival step into the target slot and set its bit in the
-// ring row of their arrival step (or join the target slice's pending list).
+// depolarisation and arrival step into the target slot and join the bucket of
+// their arrival step at the target slice (or the slice's pending list).
 __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, const nc_edge& ed, int32_t s,
                                              uint32_t& n_acc) {
   // a spike still in flight or in its release window (:728)
@@ -779,17 +764,13 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   const uint32_t d = ed.tq_d & 0xffu;
   const int32_t arr = s + (int32_t)d;
   const uint32_t tq = ed.tq_d >> 8;
+  // the bucket place is taken before the slot is read, so that the two round
+  // trips overlap
+  size_t bi = 0;
+  uint32_t at = NC_BCAP;
   if (d <= NC_NB) {
-    // the slot's bit in the ring row of its arrival step (no value returned:
-    // the wave does not wait for it)
-    __hip_atomic_fetch_or(a.ring + (size_t)((uint32_t)arr & (NC_NB - 1u)) * a.ring_words + (ed.mwb >> 5),
-                          1u << (ed.mwb & 31u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  } else {
-    // the pending list of its window, at the target slice's slot range
-    const uint32_t tbase = a.slice_off[tq];
-    const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
-    const uint32_t at = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
-    (plist ? a.pend1 : a.pend0)[tbase + at] = ((ed.j - tbase) << 8) | ((uint32_t)arr & 255u);
+    bi = (size_t)tq * NC_NB + ((uint32_t)arr & (NC_NB - 1u));
+    at = atomicAdd(&a.bcnt[bi], 1u);
   }
   // The record's previous arrival (it has arrived: the latch released) moves
   // to last_arr, which stands for lastSpikeArrival until the new one arrives.
@@ -801,6 +782,15 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   n.last_arr = prev != NC_NEVER ? prev : r.last_arr;
   n.arr_inh = arr * 2 + (r.arr_inh & 1);
   a.slots[ed.j] = n;
+  if (at < NC_BCAP) {
+    a.bent[bi * NC_BCAP + at] = make_uint2(ed.rel, __float_as_uint(n.depol));
+  } else {
+    // the pending list of its window, at the target slice's slot range
+    const uint32_t tbase = ed.j - ed.rel;
+    const uint32_t plist = ((uint32_t)arr >> NC_PEND_LOG_W) & 1u;
+    const uint32_t pt = atomicAdd(reinterpret_cast<uint32_t*>(&a.pend_cnt[tq]) + plist, 1u);
+    (plist ? a.pend1 : a.pend0)[tbase + pt] = (ed.rel << 8) | ((uint32_t)arr & 255u);
+  }
   a.edges[e].arr = arr;
   ++n_acc;
 }
