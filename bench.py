@@ -87,8 +87,9 @@ def fanout_kernel_bytes(c0, c1, steps):
 
 
 def copy_calibration(mib):
-    """A plain streaming kernel of known size (torch.neg over `mib` MiB of
-    float32 into a second buffer: reads and writes exactly that many bytes),
+    """Two plain streaming kernels of known size (torch.neg over `mib` MiB of
+    float32 into a second buffer, then a device-to-device copy of the same
+    size: each reads and writes exactly that many bytes), five launches each,
     for scaling the FETCH_SIZE / WRITE_SIZE counters to bytes."""
     import torch
     n = mib * (1 << 20) // 4
@@ -96,6 +97,9 @@ def copy_calibration(mib):
     dst = torch.empty_like(src)
     for _ in range(5):
         torch.neg(src, out=dst)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        dst.copy_(src)
     torch.cuda.synchronize()
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4,
                       "launches": 5}), flush=True)

@@ -382,6 +382,9 @@ struct NetRec {
   uint32_t pos(uint32_t id) const { return q0 * 64 + maps->pos_of_id[id]; }
   float lr = 1.f, pre_f = 0.13f, post_f = 0.30f;
   uint32_t steps_done = 0, steps_left = UINT32_MAX;
+  // first step after the last host weight write: spikes fired before it
+  // carry a stale weight in their arrival-bucket entries
+  uint32_t whost = 0;
   uint32_t grp0 = 0, ngrp = 0, thr0 = 0, nthr = 0;
   nc_drive_spec drive{};
   bool has_drive = false;
@@ -1157,6 +1160,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
       return fail(NC_ERR_RANGE, "step count limit reached");
     nd.base_step = (int32_t)r.steps_done;
     nd.remaining = (int32_t)run;
+    nd.whost = (int32_t)r.whost;
     nd.lr = r.lr;
     nd.pre_f = r.pre_f;
     nd.post_f = r.post_f;
@@ -1568,6 +1572,7 @@ extern "C" nc_status nc_write_weights(nc_sim* sim, int32_t net_id, const float* 
     x.w = w[k];
     x.arr_inh = (x.arr_inh & ~1) | ((w[k] < 0.0f) ? 1 : 0);
   }
+  r->whost = r->steps_done + 1;
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice,
                         sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));
@@ -1590,6 +1595,7 @@ extern "C" nc_status nc_set_weights(nc_sim* sim, int32_t net_id, uint64_t n, con
     x.w = w[i];
     x.arr_inh = (x.arr_inh & ~1) | ((w[i] < 0.0f) ? 1 : 0);
   }
+  r->whost = r->steps_done + 1;
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice,
                         sim->stream));
   NC_HIP(hipStreamSynchronize(sim->stream));

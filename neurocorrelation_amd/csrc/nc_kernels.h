@@ -41,7 +41,9 @@
 #define NC_PEND_LOG_W 4
 // Spikes with delays up to NC_NB steps go to an arrival bucket instead: per
 // slice, one bucket per arrival step modulo NC_NB, of up to NC_BCAP entries
-// (slot - first slot of the slice, latched depolarisation).  Fan-out takes
+// (slot - first slot of the slice | (delay - 1) << 24 | sign class << 31,
+// weight at the fire, whose product with 10.4 is the latched
+// depolarisation).  Fan-out takes
 // the place with an atomic on the bucket's count; the step of arrival a reads
 // bucket a % NC_NB of each slice -- exactly its arrivals, with what they
 // contribute -- and empties it.  A spike that finds its bucket full, or
@@ -82,7 +84,8 @@ struct nc_netdesc {
   uint32_t drive_lo, drive_hi;  // drive seed
   uint32_t bg_thr;              // background threshold on a u32 draw (0 = off)
   uint32_t bg_lo, bg_hi;        // background seed
-  uint32_t pad[6];
+  int32_t whost;                // first step after the last host weight write
+  uint32_t pad[5];
 };
 
 // One record per active slice (one wave).
@@ -159,7 +162,7 @@ struct nc_step_args {
   uint32_t* pend1;         // pending list 1
   uint2* pend_cnt;         // per slice: entries in pending list 0, in list 1
   uint32_t* bcnt;          // per slice: NC_NB arrival-bucket counts (beyond NC_BCAP: the rest overflowed)
-  uint2* bent;             // per slice: NC_NB buckets of NC_BCAP entries {slot - first slot, depol bits}
+  uint2* bent;             // per slice: NC_NB buckets of NC_BCAP entries {slot | delay | class, weight bits}
   uint4* acache;           // per position: (depol bits, arrival) of its first two active slots
   const uint32_t* ev_pos;  // host events of this step
   const uint8_t* ev_kind;
@@ -596,29 +599,33 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   }
 
   // Spikes arriving at s (Synapse::run, :718-726): the bucket's entries,
-  // which carry the latched depolarisation, then the pending list's
-  // arrivals, whose depolarisation is read from the slot record.  Each one
-  // takes its pre-side STDP now -- synapse trace powf(decay, 0) == 1 -> 0,
-  // target trace of lastFire after this step; the record already holds the
-  // arrival as lastSpikeArrival, and is read only when the target trace is
-  // not 0 -- and joins the masks (contributing from s+1) and the next cache.
-  // An entry belongs to the lane that owns its slot: six ballots on the bits
-  // of the target lane give each lane the set of entries that are its own.
+  // which carry the weight at the fire (hence the latched depolarisation),
+  // the delay and the sign class, then the pending list's arrivals, whose
+  // state is read from the slot record.  Each one takes its pre-side STDP
+  // now -- synapse trace powf(decay, 0) == 1 -> 0, target trace of lastFire
+  // after this step; the record already holds the arrival as
+  // lastSpikeArrival -- and joins the masks (contributing from s+1) and the
+  // next cache.  The weight is the entry's while nothing can have changed it
+  // since the fire (the target has not fired since, and no host write came
+  // after it); else, and for pending-list spikes, the record is read.  An
+  // entry belongs to the lane that owns its slot: six ballots on the bits of
+  // the target lane give each lane the set of entries that are its own.
   const float tr_t = nc_lookup(tpost, a.len_post, s - lf);
   uint32_t arrw = 0;  // which of mask words 0-3 took arrivals
   // batch 0: the bucket's entries; batch b > 0: pending entries 64 (b - 1) ..
   const uint32_t n_batches = 1u + (pend_n + 63u) / 64u;
   uint32_t kept = 0;  // pending entries kept (compacted in place: write index <= read index)
   for (uint32_t bt = (b_take ? 0u : 1u); bt < n_batches; ++bt) {
-    const bool known = bt == 0u;  // bucket entries carry the depolarisation
-    uint32_t rel = be.x, dbits = be.y, e = 0u, t = 0u;
+    const bool known = bt == 0u;  // bucket entries carry the weight, delay and class
+    uint32_t rel = be.x & 0xFFFFFFu, info = be.x >> 24, wbits = be.y, e = 0u, t = 0u;
     bool ev = lane < b_take;
     if (!known) {
       t = (bt - 1u) * 64u + lane;
       e = t < pend_n ? pl[t] : 0u;
       ev = t < pend_n && (e & 255u) == ((uint32_t)s & 255u);
       rel = e >> 8;
-      dbits = 0u;
+      info = 0u;
+      wbits = 0u;
     }
     // entry (slot - first slot of the slice) -> the lane that owns the slot
     const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
@@ -632,7 +639,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     while (__ballot(mine != 0ull)) {
       const uint32_t src = mine ? (uint32_t)__builtin_ctzll(mine) : lane;
       const uint32_t k = (uint32_t)__shfl((int)kk, (int)src);
-      uint32_t dk = (uint32_t)__shfl((int)dbits, (int)src);
+      const uint32_t inf = (uint32_t)__shfl((int)info, (int)src);
+      const uint32_t wb = (uint32_t)__shfl((int)wbits, (int)src);
       if (mine) {
         mine &= mine - 1;
         const uint32_t w = k >> 5, bit = 1u << (k & 31u);
@@ -646,17 +654,25 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
           ++n_mwb;
         }
         sm_keep |= 1u << (G == 1u ? w : w / G);
-        if (tr_t != 0.0f || !known) {
-          const uint32_t j = nc_slot_of(rec.slot_off, lane, k);
+        const uint32_t j = nc_slot_of(rec.slot_off, lane, k);
+        const int32_t s_fire = s - (int32_t)((inf & 63u) + 1u);
+        const bool fresh = known && lf <= s_fire && s_fire >= nd.whost;
+        // the weight now and its sign class: the entry's, or the record's
+        float w_now = __uint_as_float(wb);
+        uint32_t inh = inf >> 7;
+        uint32_t dk = __float_as_uint(a.depol_k * w_now);  // Synapse::fire (:730-732), as fan-out made it
+        if (!fresh && (tr_t != 0.0f || !known)) {
           const nc_slot r = a.slots[j];
           ++n_arec;
+          w_now = r.w;
+          inh = (uint32_t)r.arr_inh & 1u;
           if (!known) dk = __float_as_uint(r.depol);
-          if (tr_t != 0.0f) {
-            const float wn = nc_stdp(r.w, (uint32_t)r.arr_inh & 1u, 0.0f, tr_t, pre_f, post_f, lr);
-            if (__float_as_uint(wn) != __float_as_uint(r.w)) {
-              a.slots[j].w = wn;
-              ++n_wset;
-            }
+        }
+        if (tr_t != 0.0f) {
+          const float wn = nc_stdp(w_now, inh, 0.0f, tr_t, pre_f, post_f, lr);
+          if (__float_as_uint(wn) != __float_as_uint(w_now)) {
+            a.slots[j].w = wn;
+            ++n_wset;
           }
         }
         nc_push(nk, nv, ncnt, k, make_uint2(dk, (uint32_t)s));
@@ -783,7 +799,8 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   n.arr_inh = arr * 2 + (r.arr_inh & 1);
   a.slots[ed.j] = n;
   if (at < NC_BCAP) {
-    a.bent[bi * NC_BCAP + at] = make_uint2(ed.rel, __float_as_uint(n.depol));
+    a.bent[bi * NC_BCAP + at] =
+        make_uint2(ed.rel | ((d - 1u) << 24) | ((uint32_t)(r.arr_inh & 1) << 31), __float_as_uint(r.w));
   } else {
     // the pending list of its window, at the target slice's slot range
     const uint32_t tbase = ed.j - ed.rel;
