@@ -70,8 +70,10 @@
 // slots come from the cache, and four entries cost more cache traffic than
 // they save in record reads (a measured 68 against 61 us per step).
 #define NC_ACACHE 2
-// Slot records fetched together in the charge and arrival passes.
-#define NC_RB 2
+// Slot records fetched together by the charge for active slots past the
+// cache (one: fewer registers, a wave more per SIMD; 57 against 59 us per step
+// of k_neuron at 10^6 neurons).
+#define NC_RB 1
 // Group spike trains are drawn for at most NC_GRING steps ahead, into a ring
 // of that many rows (a call of more steps draws them chunk by chunk).
 #define NC_GRING 1024u
@@ -816,10 +818,16 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
                                                int32_t ci) {
   const uint32_t lane = threadIdx.x & 63;
   uint32_t n_acc = 0, n_out = 0;
-  // Wave w takes list w % NC_FLISTS, entries w / NC_FLISTS, + per, ... (64
-  // records per load).
-  const uint32_t per = nwaves / NC_FLISTS;
-  const uint32_t lst = wave % NC_FLISTS, first = wave / NC_FLISTS;
+  // Wave w takes list w % NC_FLISTS; when the grid has at least two waves
+  // per list, two waves share each record, one taking the edges of even
+  // 64-edge groups and the other those of odd ones (a fired slice's edges,
+  // about a hundred at 1% activity, then take one pass of one edge per lane
+  // in each of two waves).  Entries first, first + per, ... (64 per load).
+  const uint32_t per_list = nwaves / NC_FLISTS;
+  const uint32_t parts = per_list >= 2u ? 2u : 1u;
+  const uint32_t per = per_list / parts;
+  const uint32_t lst = wave % NC_FLISTS, rest = wave / NC_FLISTS;
+  const uint32_t part = rest % parts, first = rest / parts;
   const uint32_t nf = first < per ? __builtin_amdgcn_readfirstlane(
                                         a.fired_cnt[((((uint32_t)ci & 1u) * NC_FLISTS) + lst) * NC_FCNT_STRIDE])
                                   : 0u;
@@ -836,7 +844,7 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
       const uint32_t fq = (uint32_t)__shfl((int)fl.w, (int)fsrc);
       const unsigned long long m = ((unsigned long long)fy << 32) | fx;
       // The out-lists of all fired lanes, flattened: lane-parallel row loads,
-      // a wave prefix sum of out-degrees, then 128 edges per pass.
+      // a wave prefix sum of out-degrees, then this wave's 64-edge groups.
       uint2 row = make_uint2(0u, 0u);
       if ((m >> lane) & 1ull) row = a.out_row[fq * 64 + lane];
       uint32_t incl = row.y;
@@ -846,32 +854,25 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
         if (lane >= (uint32_t)o) incl += v;
       }
       const uint32_t total = (uint32_t)__shfl((int)incl, 63);
-      n_out += total;
-      for (uint32_t base = 0; base < total; base += 128) {
-        uint32_t ie[2], src[2];
-        nc_edge ed[2];
+      if (part == 0u) n_out += total;
+      for (uint32_t base = 64u * part; base < total; base += 64u * parts) {
+        const uint32_t i = base + lane;
+        // source lane: the first lane whose inclusive prefix exceeds i
+        uint32_t lo = 0;
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const uint32_t i = base + 64u * h + lane;
-          // source lane: the first lane whose inclusive prefix exceeds i
-          uint32_t lo = 0;
-#pragma unroll
-          for (uint32_t b = 32; b >= 1; b >>= 1) {
-            const uint32_t v = (uint32_t)__shfl((int)incl, (int)(lo + b - 1));
-            if (v <= i) lo += b;
-          }
-          src[h] = lo > 63 ? 63 : lo;
-          const uint32_t rx = (uint32_t)__shfl((int)row.x, (int)src[h]);
-          const uint32_t ry = (uint32_t)__shfl((int)row.y, (int)src[h]);
-          const uint32_t ic = (uint32_t)__shfl((int)incl, (int)src[h]);
-          ie[h] = i < total ? rx + (i - (ic - ry)) : ~0u;
+        for (uint32_t b = 32; b >= 1; b >>= 1) {
+          const uint32_t v = (uint32_t)__shfl((int)incl, (int)(lo + b - 1));
+          if (v <= i) lo += b;
         }
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-          if (ie[h] != ~0u) ed[h] = a.edges[ie[h]];
-#pragma unroll
-        for (int h = 0; h < 2; ++h)
-          if (ie[h] != ~0u) nc_fire_edge(a, ie[h], ed[h], s, n_acc);
+        const uint32_t src = lo > 63 ? 63 : lo;
+        const uint32_t rx = (uint32_t)__shfl((int)row.x, (int)src);
+        const uint32_t ry = (uint32_t)__shfl((int)row.y, (int)src);
+        const uint32_t ic = (uint32_t)__shfl((int)incl, (int)src);
+        if (i < total) {
+          const uint32_t ie = rx + (i - (ic - ry));
+          const nc_edge ed = a.edges[ie];
+          nc_fire_edge(a, ie, ed, s, n_acc);
+        }
       }
     }
   }

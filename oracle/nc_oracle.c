@@ -565,6 +565,15 @@ OR_EXPORT void or_sim_counters(const or_sim* o, uint64_t out[6]) {
 
 OR_EXPORT void or_sim_reset_activities(or_sim* o) { memset(o->firings, 0, 4ull * o->n); }
 
+/* Synapse::setWeight (src/NeuCor.cpp:576-579) on the first n synapses (in-CSR
+ * order): the weight, and the sign class re-derived from it. */
+OR_EXPORT void or_sim_set_weights(or_sim* o, const float* w, uint64_t n) {
+  for (uint64_t k = 0; k < n && k < o->S; ++k) {
+    o->w[k] = w[k];
+    o->inh[k] = w[k] < 0.0f ? 1 : 0;
+  }
+}
+
 /* Closed-form pieces, for table checks. */
 OR_EXPORT float or_trace(float decay, float x) { return powf(decay, x); }
 OR_EXPORT float or_ap_value(float dt_since_fire) {

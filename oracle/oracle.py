@@ -45,6 +45,7 @@ def lib():
         L.or_sim_create.argtypes = [C.c_uint32, C.c_uint64, P, P, P, P, P] + [C.c_float] * 6
         L.or_sim_free.argtypes = [P]
         L.or_sim_set_params.argtypes = [P, C.c_float, C.c_float, C.c_float]
+        L.or_sim_set_weights.argtypes = [P, P, C.c_uint64]
         L.or_sim_set_drive.argtypes = [P, C.c_uint32, P, P, C.c_uint32, C.c_uint32, C.c_float, C.c_float,
                                        C.c_float, C.c_float, C.c_uint64]
         L.or_sim_push_forced.argtypes = [P, C.c_uint32, P, P, P]
@@ -172,6 +173,10 @@ class Sim:
         lib().or_sim_counters(self._h, _p(out))
         return dict(zip(["ties_latch", "ties_weight", "arrivals", "fires", "accepted", "first_tie"],
                         (int(x) for x in out)))
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        lib().or_sim_set_weights(self._h, _p(w), C.c_uint64(len(w)))
 
     def reset_activities(self):
         lib().or_sim_reset_activities(self._h)

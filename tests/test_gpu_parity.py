@@ -253,10 +253,26 @@ def test_live_params_weights_and_reset(oracle_mod):
         o.step(333)
         assert_bits(sim.weights(nid, net.n_syn), o.state()["w"], "w")
         assert_bits(sim.potential(nid, net.n), o.state()["V"], "V")
+    # the weights are written with spikes in flight (their arrival entries
+    # carry the old weights), fired by the eight widest sources one step
+    # before; the run goes on as in the oracle with the same write
+    top = np.argsort(-np.bincount(ex["in_src"], minlength=net.n), kind="stable")[:8]
+    sim.push_forced(nid, [1] * 8, top, [0] * 8)
+    o.push_forced([1] * 8, top, [0] * 8)
+    sim.step(1)
+    o.step(1)
+    arr, _, _ = sim.synapse_state(nid, net.n_syn)
+    assert (arr > sim.progress(nid)[0]).sum() >= 8
     w = sim.weights(nid, net.n_syn)
+    assert_bits(w, o.state()["w"], "w before the write")
     w2 = np.where(np.arange(len(w)) % 3 == 0, -np.abs(w), w).astype(np.float32)
     sim.write_weights(nid, w2)
+    o.set_weights(w2)
     assert_bits(sim.weights(nid, net.n_syn), w2, "written w")
+    sim.step(333)
+    o.step(333)
+    assert_bits(sim.weights(nid, net.n_syn), o.state()["w"], "w after the write")
+    assert_bits(sim.potential(nid, net.n), o.state()["V"], "V after the write")
     sim.reset_activities(nid)
     assert sim.firings(nid, net.n).sum() == 0
 
