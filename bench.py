@@ -32,9 +32,10 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
     the engine's event counters over the timed steps:
       20 B per neuron        in-degree/flags + input group 8, V read+write 8,
                              lastFire read 4
-      48 B per 64 neurons    active-slice record 16, pending counts 8,
-                             fired-lane record write 16, arrival-bucket count
-                             read + reset 8
+      32 B per 64 neurons    active-slice record 16, pending counts 8,
+                             arrival-bucket count read + reset 8
+      16 B per fired-list    {mask lo, mask hi, step, slice | share} written
+           record            (a share of a fired slice's out-edges)
        4 B per mask word     read (words a summary bit flags) and 4 B per
                              word written back
        8 B per active slot   (depolarisation, arrival) from the active-slot
@@ -46,8 +47,9 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
                              spike came through a pending list)
        4 B per weight        written by the arrival's pre-side STDP
        8 B per cache entry   written (lanes whose active set changed)
-      20 B per in-synapse    post-side STDP of a fired neuron: record read 16,
-           of a fired neuron w write 4
+      16 B per in-synapse    post-side STDP of a fired neuron: record read
+           of a fired neuron
+       4 B per weight        written by post-side STDP (bits changed)
       12 B per fired neuron  firings read+write 8, lastFire write 4
        4 B per pending entry scanned (spikes whose delays exceed NC_NB steps or
                              found their bucket full, in the list of the step's
@@ -55,23 +57,26 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
     d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree",
                                                "pending_scanned", "mask_words_read", "mask_words_written",
                                                "bucket_entries", "records_read", "arrival_weight_writes",
-                                               "cache_entries_written", "arrival_records_read")}
+                                               "cache_entries_written", "arrival_records_read", "fired_records",
+                                               "post_weight_writes")}
     cached = d["active_slot_steps"] - d["records_read"]
-    b = (20.0 * n_neurons + 48.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
+    b = (20.0 * n_neurons + 32.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
          + 8.0 * cached + 16.0 * d["records_read"] + 8.0 * d["bucket_entries"] + 16.0 * d["arrival_records_read"]
-         + 4.0 * d["arrival_weight_writes"] + 8.0 * d["cache_entries_written"] + 20.0 * d["fired_in_degree"]
-         + 12.0 * d["fired"] + 4.0 * d["pending_scanned"])
+         + 4.0 * d["arrival_weight_writes"] + 8.0 * d["cache_entries_written"] + 16.0 * d["fired_in_degree"]
+         + 4.0 * d["post_weight_writes"] + 16.0 * d["fired_records"] + 12.0 * d["fired"] + 4.0 * d["pending_scanned"])
     return b, {"A": d["active_slot_steps"], "A_records": d["records_read"], "R": d["arrivals"],
                "R_bucket": d["bucket_entries"], "R_records": d["arrival_records_read"],
                "R_w": d["arrival_weight_writes"], "cache_w": d["cache_entries_written"], "F": d["fired"],
-               "F_in": d["fired_in_degree"], "pending": d["pending_scanned"], "mask_words": d["mask_words_read"],
+               "F_in": d["fired_in_degree"], "F_in_w": d["post_weight_writes"], "records": d["fired_records"],
+               "pending": d["pending_scanned"], "mask_words": d["mask_words_read"],
                "mask_words_written": d["mask_words_written"]}
 
 
 def fanout_kernel_bytes(c0, c1, steps):
     """Algorithmic bytes per launch of the fan-out kernel (k_fanout), from the
     engine's event counters over the timed steps:
-      16 B per fired-slice record   {mask lo, mask hi, step, slice} read
+      16 B per fired-list record    {mask lo, mask hi, step, slice | share}
+                                    read
        8 B per fired neuron         out-row head {first edge, out-degree}
       16 B per out-edge of a        edge record {target slot, target slice |
            fired neuron             delay, latched arrival, target slice base}

@@ -273,17 +273,20 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * [8] pending-list entries scanned (spikes whose delays exceed the
  * arrival ring), [9] mask words of every stepped neuron, ceil(in-degree /
  * 32) each, [10] active-mask words
- * written back, [11] fired-slice records (64-neuron slices with at least one
- * fired neuron, summed over steps), [12] active-mask words read (those a
+ * written back, [11] fired-list records (per 64-neuron slice with fired
+ * neurons, one per share of its out-edges given to a fan-out wave: up to 16,
+ * by fired neurons times their network's mean out-degree in 64-edge groups,
+ * summed over steps), [12] active-mask words read (those a
  * summary bit flags), [13] arrival-bucket entries taken, [14] slot records
  * read by the charge (active slots past the active-slot cache), [15] weights
  * written at arrival (pre-side STDP that changed the weight), [16]
  * active-slot cache entries written, [17] slot records read at arrival (the
- * target fired recently, or the spike came through a pending list).
+ * target fired recently, or the spike came through a pending list), [18]
+ * weights written by post-side STDP (those whose bits changed).
  * [0], [1] and [9] are kept by the host; the others are device counters that
  * advance only while counting is on (nc_set_counting).
  * Writes the first n (at most NC_NUM_COUNTERS) into out. */
-#define NC_NUM_COUNTERS 18
+#define NC_NUM_COUNTERS 19
 nc_status nc_read_counters(nc_sim* sim, uint64_t* out, uint32_t n);
 
 /* Checkpoint of one network's dynamic state.  No counterpart in the

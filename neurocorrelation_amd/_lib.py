@@ -11,7 +11,7 @@ LIB_PATH = os.path.join(HERE, "libneucor_amd.so")
 
 NC_OK = 0
 NC_NEVER_STEP = -(1 << 30)
-NC_NUM_COUNTERS = 18
+NC_NUM_COUNTERS = 19
 STATUS = {
     0: "NC_OK", 1: "NC_ERR_ARG", 2: "NC_ERR_RANGE", 3: "NC_ERR_HIP", 4: "NC_ERR_OOM",
     5: "NC_ERR_CAPACITY", 6: "NC_ERR_STATE", 7: "NC_ERR_NO_DEVICE",

@@ -674,7 +674,8 @@ extern "C" nc_status nc_sim_footprint(uint64_t n_neurons, uint64_t n_syn, uint64
   const uint64_t mwords = slots / 32 + 64 * Q;
   *bytes = P * (4 + 12 + 8 + 4 + 8) + P / 4                // asum, V/lastFire/firings, ndw, nid, out_row, forced
            + P * sizeof(uint4)                             // active-slot cache
-           + Q * (8 + 4 + 4 + sizeof(nc_slice_rec) + 16)  // pend_cnt, slice_off, slice_net, active, fired
+           + Q * (8 + 4 + 4 + sizeof(nc_slice_rec))       // pend_cnt, slice_off, slice_net, active
+           + (Q + NC_FLISTS * 8) * NC_FSPLIT * 16          // fired lists
            + Q * NC_NB * (4ull + 8ull * NC_BCAP) + Q * 4     // arrival buckets, slice mask offsets
            + slots * (sizeof(nc_slot) + 8) + mwords * 4    // slot records, pending lists, active masks
            + (n_syn + 64) * sizeof(nc_edge)                // out-edges
@@ -717,6 +718,9 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
   if (spec->device < 0 || spec->device >= ndev) return fail(NC_ERR_RANGE, "device index out of range");
   if (spec->max_neurons == 0 || spec->max_nets == 0) return fail(NC_ERR_ARG, "zero capacity");
   if (spec->max_synapses >= (1ull << 32) - (1ull << 24)) return fail(NC_ERR_ARG, "max_synapses too large");
+  // fired-list records carry the slice index in NC_FSLICE_BITS bits
+  if ((uint64_t)spec->max_neurons / 64 + 2ull * spec->max_nets >= (1ull << NC_FSLICE_BITS))
+    return fail(NC_ERR_ARG, "max_neurons too large");
   nc_sim* sim = new (std::nothrow) nc_sim();
   if (!sim) return fail(NC_ERR_OOM, "host allocation failed");
   // destroys the half-built simulation on a failure or an exception
@@ -767,14 +771,15 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMalloc(&sim->active, Q * sizeof(nc_slice_rec)), "hipMalloc active");
     H(hipMalloc(&sim->forced_in, (P / 32) * 4), "hipMalloc forced_in");
     H(hipMalloc(&sim->forced_bg, (P / 32) * 4), "hipMalloc forced_bg");
-    H(hipMalloc(&sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)), "hipMalloc fired");
+    H(hipMalloc(&sim->fired, (Q + NC_FLISTS * 8ull) * NC_FSPLIT * sizeof(uint4)), "hipMalloc fired");
     H(hipMalloc(&sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4), "hipMalloc fired_cnt");
 
     H(hipMalloc(&sim->raster_cnt, 64), "hipMalloc raster_cnt");
     if (sim->raster_cap) H(hipMalloc(&sim->raster, 8ull * sim->raster_cap), "hipMalloc raster");
     sim->ctr_rows = Q + 2;
     // word 0: chunk offset; word 32: grid-barrier arrivals of the call;
-    // word 64: a grid barrier timed out (sticky until reported)
+    // word 64: a grid barrier timed out (sticky until reported); word 128:
+    // zero, the group-spike-train word read when there are no groups
     H(hipMalloc(&sim->d_chunk_off, 1024), "hipMalloc chunk_off");
     H(hipMalloc(&sim->ctr, sim->ctr_rows * NC_NCTR * 8), "hipMalloc ctr");
     // one more, never written: the (never live) network of hole records
@@ -786,7 +791,7 @@ extern "C" nc_status nc_sim_create(const nc_sim_spec* spec, nc_sim** out) try {
     H(hipMemsetAsync(sim->ndw, 0, P * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_in, 0, (P / 32) * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->forced_bg, 0, (P / 32) * 4, sim->stream), "memset");
-    H(hipMemsetAsync(sim->fired, 0, (Q + NC_FLISTS * 8ull) * sizeof(uint4), sim->stream), "memset");
+    H(hipMemsetAsync(sim->fired, 0, (Q + NC_FLISTS * 8ull) * NC_FSPLIT * sizeof(uint4), sim->stream), "memset");
     H(hipMemsetAsync(sim->fired_cnt, 0, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4, sim->stream), "memset");
     H(hipMemsetAsync(sim->pend_cnt, 0, Q * sizeof(uint2), sim->stream), "memset");
     H(hipMemsetAsync(sim->bcnt, 0, Q * NC_NB * 4ull, sim->stream), "memset");
@@ -1229,19 +1234,36 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   a.thr = sim->d_thr.p;
   a.n_groups = (uint32_t)sim->groups.size();
   a.gwords = (a.n_groups + 31) / 32;
-  a.gfire = sim->d_gfire.p;
+  // k_neuron reads a group-spike-train word for every neuron (word 0 for
+  // neurons that are not inputs): a zero word stands in for an empty ring
+  a.gfire = sim->d_gfire.p ? sim->d_gfire.p : sim->d_chunk_off + 128;
   // nc_sim_add sizes the drive tables and the group-train ring; nothing here allocates
   if (a.n_groups && (sim->d_groups.n < a.n_groups || sim->d_gfire.n < (size_t)NC_GRING * a.gwords))
     return fail(NC_ERR_STATE, "drive tables were not placed");
   const uint32_t wpb = sim->cfg.waves_per_block;
   const uint32_t grid = (a.n_active_slices + wpb - 1) / wpb;
-  // A fired list holds at most fcap records (one per k_neuron wave of its
-  // residue).  k_fanout gives each list as many readers as that, so that a
-  // record is one wave's work, up to fanout_waves waves in all (default
-  // 16384, 64 per CU): then each wave takes every (waves / 64)-th record.
-  a.fcap = (grid * wpb + NC_FLISTS - 1) / NC_FLISTS;
-  const uint32_t fan_waves = std::min<uint32_t>(sim->cfg.fanout_waves ? sim->cfg.fanout_waves : 16384u,
-                                                a.fcap * NC_FLISTS);
+  // A fired list holds at most fcap records (up to NC_FSPLIT per k_neuron
+  // wave of its residue).  k_fanout gives the lists about one reader per
+  // record, up to fanout_waves waves in all (default 16384, 64 per CU): then
+  // each wave takes every (waves / 64)-th record of its list.  A fired
+  // slice's out-edges are split over several records only when that grid has
+  // a wave per active slice or more (else every wave has records enough):
+  // about one record per 64 edges at the active networks' mean out-degree.
+  a.fcap = (grid * wpb + NC_FLISTS - 1) / NC_FLISTS * NC_FSPLIT;
+  uint64_t syn_active = 0, n_active = 0;
+  for (size_t i = 0; i < sim->nets.size(); ++i)
+    if (sim->nets[i].live && nds[i].remaining > 0) {
+      syn_active += sim->nets[i].S;
+      n_active += sim->nets[i].n;
+    }
+  const uint64_t wave_cap = std::min<uint64_t>(sim->cfg.fanout_waves ? sim->cfg.fanout_waves : 16384u,
+                                               (uint64_t)a.fcap * NC_FLISTS);
+  const bool split = wave_cap >= a.n_active_slices && n_active > 0;
+  a.fsplit_od = split ? (uint32_t)std::min<uint64_t>(1u << 20, (syn_active + n_active - 1) / n_active) : 0u;
+  const uint64_t rec_bound =
+      (uint64_t)grid * wpb + (split ? (uint64_t)grid * wpb + syn_active / 64 : 0u);
+  const uint32_t fan_waves =
+      (uint32_t)std::min<uint64_t>(wave_cap, (rec_bound + NC_FLISTS - 1) / NC_FLISTS * NC_FLISTS);
   const uint32_t fan_grid = std::max<uint32_t>((fan_waves + wpb - 1) / wpb, (NC_FLISTS + wpb - 1) / wpb);
   const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
   const void* fn_neuron = nullptr;
@@ -1942,7 +1964,7 @@ nc_status state_save(nc_sim* sim, StateCopy* c) {
       {sim->bcnt, Q * NC_NB * 4ull},
       {sim->bent, Q * NC_NB * NC_BCAP * sizeof(uint2)},
       {sim->acache, P * sizeof(uint4)},
-      {sim->fired, (Q + NC_FLISTS * 8ull) * sizeof(uint4)},
+      {sim->fired, (Q + NC_FLISTS * 8ull) * NC_FSPLIT * sizeof(uint4)},
       {sim->fired_cnt, 2ull * NC_FLISTS * NC_FCNT_STRIDE * 4},
       {sim->forced_in, (P / 32) * 4},
       {sim->forced_bg, (P / 32) * 4},

@@ -13,7 +13,8 @@
 //             list -- each by the lane that owns the slot (pre-side STDP,
 //             active bits, cache)
 //   k_fanout  waves sharing the step's records of slices with fired lanes
-//             (64 lists, one record per wave below a cap): Synapse::fire on
+//             (64 lists; a slice's out-edges split over up to NC_FSPLIT
+//             records, one record per wave below a cap): Synapse::fire on
 //             the fired lanes' outgoing synapses (source-order latch copy,
 //             out->in slot map), appending accepted spikes to the target
 //             slice's bucket of their arrival step, or, past NC_NB steps or
@@ -53,13 +54,23 @@
 // Slices with fired lanes go to one of NC_FLISTS lists (by wave index), so
 // that the appends of a step spread over that many counters.
 #define NC_FLISTS 64
+// A slice with fired lanes puts its fan-out work -- the fired lanes'
+// out-edges, flattened, in 64-edge groups -- into n records of its list:
+// record i takes groups i, i + n, i + 2n, ...  When the fan-out grid has a
+// wave per active slice or more, n = min(NC_FSPLIT, the groups the fired
+// lanes' edges fill at the active networks' mean out-degree), so that a burst
+// of fired lanes in one slice is shared by up to NC_FSPLIT fan-out waves;
+// with fewer waves than slices every wave has records enough and n = 1.  A
+// record is {mask lo, mask hi, step, slice | i << 24 | (n - 1) << 28}.
+#define NC_FSPLIT 16u
+#define NC_FSLICE_BITS 24
 #define NC_FCNT_STRIDE 32  // words between two counters (one cache line each)
 // In-slots are stored chunk-major: NC_CHUNK consecutive slots of a lane (one
 // 128-byte line of 16-byte records), then the next lane's.
 #define NC_CHUNK_LOG 3
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
-#define NC_NCTR 18  // counter columns (device-side: 2-8, 10-17)
+#define NC_NCTR 19  // counter columns (device-side: 2-8, 10-18)
 // Each lane keeps (depolarisation, arrival) of its first NC_ACACHE active
 // slots, in slot order, in a per-position cache entry (16 bytes: the pairs
 // of position p at acache[p]): the charge reads those with one coalesced
@@ -141,6 +152,7 @@ struct nc_step_args {
   uint4* fired;  // slices with fired lanes in NC_FLISTS lists (list b at b * fcap): mask lo/hi, step, slice
   uint32_t* fired_cnt;  // list lengths: [parity of the step in the call][list], NC_FCNT_STRIDE apart
   uint32_t fcap;        // entries per list
+  uint32_t fsplit_od;   // mean out-degree for splitting fan-out records (0: no split)
   uint32_t* raster;           // (pos, step) pairs
   uint32_t* raster_cnt;
   uint32_t raster_cap;
@@ -385,13 +397,25 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   int32_t lf = a.last_fire[pos];
   const uint2 pc = a.pend_cnt[q];
   uint32_t sm = a.asum[pos];
+  // Loaded by every lane, with no branch between these loads and their uses,
+  // so that the wave waits for each one alone (a load under a branch makes
+  // the wait before the next use cover every load in flight): the active-slot
+  // cache (read only while the walk is within it), the id (used by the
+  // background draw), all NC_BCAP entry places of the arrival bucket of step
+  // s (the first b_n hold this step's arrivals: spikes with delays up to NC_NB
+  // that found room in it; the others wait in the pending list of s's arrival
+  // window) and the group-spike-train word of an input's group (word 0 for
+  // the others; gfire points at a zero word when there are no groups).
+  const uint4 cache = a.acache[pos];
+  const uint32_t my_nid = a.nid[pos];
   const nc_netdesc nd = a.nets[rec.net];
   const bool live = wave_in && ci < nd.remaining;
   const int32_t s = nd.base_step + ci + 1;
-  // The arrival bucket of step s (spikes with delays up to NC_NB that found
-  // room in it) and the pending list of s's arrival window (the others).
   const size_t bi = (size_t)q * NC_NB + ((uint32_t)s & (NC_NB - 1u));
   uint32_t b_n = a.bcnt[bi];
+  const uint2 be = a.bent[bi * NC_BCAP + lane];
+  const uint32_t gword = a.gfire[(size_t)((uint32_t)ci & (NC_GRING - 1u)) * a.gwords +
+                                 ((nw.x & NC_DEG_INPUT) ? (nw.y >> 5) : 0u)];
   const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
   uint32_t pend_n = plist ? pc.y : pc.x;
   if (!live) {
@@ -409,20 +433,11 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   const uint32_t nmw = (deg + 31) >> 5;
   const uint32_t G = nmw <= 32 ? 1u : (nmw + 31) >> 5;  // mask words per summary bit
   uint32_t* const mrow = a.amask + rec.mask_off + lane;
-  // The bucket's entries, fetched now and taken up after the fire; the
-  // bucket is emptied for its next use (step s + NC_NB).
+  // The bucket's entries are taken up after the fire.
   const uint32_t b_take = b_n < NC_BCAP ? b_n : NC_BCAP;
-  uint2 be = make_uint2(0u, 0u);
-  if (lane < b_take) be = a.bent[bi * NC_BCAP + lane];
-  if (b_n && lane == 0) a.bcnt[bi] = 0u;
   // The lane's cache of (depolarisation, arrival) of its first NC_ACACHE
-  // active slots in slot order, read when the summary says it has any.
-  uint2 ac[NC_ACACHE] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
-  if (sm) {
-    const uint4 c = a.acache[pos];
-    ac[0] = make_uint2(c.x, c.y);
-    ac[1] = make_uint2(c.z, c.w);
-  }
+  // active slots in slot order.
+  const uint2 ac[NC_ACACHE] = {make_uint2(cache.x, cache.y), make_uint2(cache.z, cache.w)};
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
   // that arrived at s-R .. s-1) in ascending source order.  A lane walks its
@@ -514,13 +529,10 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
 
   // Forced fires: group spike trains, background, host events.
   bool f_in = false, f_bg = false;
-  if (valid && (dw & NC_DEG_INPUT)) {
-    const uint32_t g = nw.y;
-    f_in = ((a.gfire[(size_t)((uint32_t)ci & (NC_GRING - 1u)) * a.gwords + (g >> 5)] >> (g & 31)) & 1u) != 0;
-  }
+  if (valid && (dw & NC_DEG_INPUT)) f_in = ((gword >> (nw.y & 31u)) & 1u) != 0;
   if (valid && nd.bg_thr) {
     const uint64_t seed = ((uint64_t)nd.bg_hi << 32) | nd.bg_lo;
-    const uint32_t u = nc_draw(seed, (uint32_t)s, a.nid[pos], NC_STREAM_BG).v[0];
+    const uint32_t u = nc_draw(seed, (uint32_t)s, my_nid, NC_STREAM_BG).v[0];
     f_bg = u < nd.bg_thr;
   }
   if (a.use_forced && live) {
@@ -553,15 +565,24 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
   {
-    // This step's lists get the fired slices; the other parity's lists,
-    // which served the previous step's fan-out, are cleared for the next.
+    // This step's lists get the fired slices' fan-out records; the other
+    // parity's lists, which served the previous step's fan-out, are cleared
+    // for the next.
     const uint32_t par = (uint32_t)ci & 1u;
     if (blk == 0 && threadIdx.x < NC_FLISTS)
       a.fired_cnt[((par ^ 1u) * NC_FLISTS + threadIdx.x) * NC_FCNT_STRIDE] = 0u;
     if (fm && lane == 0) {
+      // the fired lanes' out-edges, in 64-edge groups, shared by n records
+      // (written by one lane: a record per lane would hold registers at the
+      // kernel's tightest point)
+      const uint32_t n_frec =
+          a.fsplit_od ? min(NC_FSPLIT, ((uint32_t)__builtin_popcountll(fm) * a.fsplit_od + 63u) >> 6) : 1u;
       const uint32_t b = wave % NC_FLISTS;  // waves running together use different lists
-      const uint32_t at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], 1u);
-      a.fired[b * a.fcap + at] = make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s, q);
+      const uint32_t at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], n_frec);
+      for (uint32_t i = 0; i < n_frec; ++i)
+        a.fired[b * a.fcap + at + i] = make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s,
+                                                  q | (i << NC_FSLICE_BITS) | ((n_frec - 1u) << (NC_FSLICE_BITS + 4)));
+      if (a.ctr) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 11], (unsigned long long)n_frec);
     }
   }
   if (fm) {
@@ -572,6 +593,7 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     // it, or a last arrival whose latch a restore cleared).  The whole wave
     // works on one fired lane's row at a time.
     unsigned long long mm = fm;
+    uint32_t n_pw = 0;  // weights written (a wave count)
     while (mm) {
       const uint32_t l = (uint32_t)__builtin_ctzll(mm);
       mm &= mm - 1;
@@ -588,13 +610,24 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
         // the record's arrival if it is a step in [1, s-1], else last_arr
         const int32_t la0 = (uint32_t)(a0 - 1) < (uint32_t)(s - 1) ? a0 : s0.last_arr;
         const int32_t la1 = (uint32_t)(a1 - 1) < (uint32_t)(s - 1) ? a1 : s1.last_arr;
-        a.slots[j0].w = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la0), 0.0f, pre_f,
-                                post_f, lr);
-        if (two)
-          a.slots[j1].w = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la1), 0.0f,
-                                  pre_f, post_f, lr);
+        // a weight whose synapse saw no recent arrival keeps its bits (the
+        // trace is below half an ulp of it): no store
+        const float w0 = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la0), 0.0f, pre_f,
+                                 post_f, lr);
+        const bool ch0 = __float_as_uint(w0) != __float_as_uint(s0.w);
+        if (ch0) a.slots[j0].w = w0;
+        bool ch1 = false;
+        if (two) {
+          const float w1 = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la1), 0.0f,
+                                   pre_f, post_f, lr);
+          ch1 = __float_as_uint(w1) != __float_as_uint(s1.w);
+          if (ch1) a.slots[j1].w = w1;
+        }
+        // counted per wave (a scalar), not per lane
+        n_pw += (uint32_t)__builtin_popcountll(__ballot(ch0)) + (uint32_t)__builtin_popcountll(__ballot(ch1));
       }
     }
+    if (a.ctr && lane == 0 && n_pw) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 18], (unsigned long long)n_pw);
     // the arrivals below read records the row passes may have written
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
@@ -691,6 +724,9 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     }
   }
   if (pend_n && lane == 0) reinterpret_cast<uint32_t*>(&a.pend_cnt[q])[plist] = kept;
+  // the bucket is emptied for its next use (step s + NC_NB, whose spikes the
+  // fan-out after this kernel may start to append)
+  if (b_n && lane == 0) a.bcnt[bi] = 0u;
 #pragma unroll
   for (int u = 0; u < 4; ++u)
     if ((arrw >> u) & 1u) {
@@ -728,7 +764,6 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     nc_count32(a.ctr, blk, 3, n_arr);
     if (lane == 0 && fm) {
       atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 4], (unsigned long long)__builtin_popcountll(fm));
-      atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 11], 1ull);
     }
     nc_count32(a.ctr, blk, 5, n_post);
     nc_count32(a.ctr, blk, 10, n_mwb);
@@ -771,8 +806,9 @@ __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
 // fired neuron: each wave takes every (nwaves / 64)-th record of one of the
-// step's 64 fired-slice lists, 64 per load, and the slices one by one; one
-// writer per synapse (its source), latch permitting.  Accepted spikes latch
+// step's 64 fired lists, 64 per load, and the records one by one (a record is
+// a share of one fired slice's out-edges); one writer per synapse (its
+// source), latch permitting.  Accepted spikes latch
 // depolarisation and arrival step into the target slot and join the bucket of
 // their arrival step at the target slice (or the slice's pending list).
 __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, const nc_edge& ed, int32_t s,
@@ -818,16 +854,10 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
                                                int32_t ci) {
   const uint32_t lane = threadIdx.x & 63;
   uint32_t n_acc = 0, n_out = 0;
-  // Wave w takes list w % NC_FLISTS; when the grid has at least two waves
-  // per list, two waves share each record, one taking the edges of even
-  // 64-edge groups and the other those of odd ones (a fired slice's edges,
-  // about a hundred at 1% activity, then take one pass of one edge per lane
-  // in each of two waves).  Entries first, first + per, ... (64 per load).
-  const uint32_t per_list = nwaves / NC_FLISTS;
-  const uint32_t parts = per_list >= 2u ? 2u : 1u;
-  const uint32_t per = per_list / parts;
-  const uint32_t lst = wave % NC_FLISTS, rest = wave / NC_FLISTS;
-  const uint32_t part = rest % parts, first = rest / parts;
+  // Wave w takes list w % NC_FLISTS, entries first, first + per, ... (64 per
+  // load), per = the waves per list.
+  const uint32_t per = nwaves / NC_FLISTS;
+  const uint32_t lst = wave % NC_FLISTS, first = wave / NC_FLISTS;
   const uint32_t nf = first < per ? __builtin_amdgcn_readfirstlane(
                                         a.fired_cnt[((((uint32_t)ci & 1u) * NC_FLISTS) + lst) * NC_FCNT_STRIDE])
                                   : 0u;
@@ -841,7 +871,9 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
       has &= has - 1;
       const uint32_t fx = (uint32_t)__shfl((int)fl.x, (int)fsrc), fy = (uint32_t)__shfl((int)fl.y, (int)fsrc);
       const int32_t s = __shfl((int)fl.z, (int)fsrc);
-      const uint32_t fq = (uint32_t)__shfl((int)fl.w, (int)fsrc);
+      const uint32_t fw = (uint32_t)__shfl((int)fl.w, (int)fsrc);
+      const uint32_t fq = fw & ((1u << NC_FSLICE_BITS) - 1u);
+      const uint32_t item = (fw >> NC_FSLICE_BITS) & 15u, n = (fw >> (NC_FSLICE_BITS + 4)) + 1u;
       const unsigned long long m = ((unsigned long long)fy << 32) | fx;
       // The out-lists of all fired lanes, flattened: lane-parallel row loads,
       // a wave prefix sum of out-degrees, then this wave's 64-edge groups.
@@ -854,8 +886,9 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
         if (lane >= (uint32_t)o) incl += v;
       }
       const uint32_t total = (uint32_t)__shfl((int)incl, 63);
-      if (part == 0u) n_out += total;
-      for (uint32_t base = 64u * part; base < total; base += 64u * parts) {
+      // this record's share: groups item, item + n, ... of the slice's n records
+      if (item == 0u) n_out += total;
+      for (uint32_t base = 64u * item; base < total; base += 64u * n) {
         const uint32_t i = base + lane;
         // source lane: the first lane whose inclusive prefix exceeds i
         uint32_t lo = 0;

@@ -223,7 +223,7 @@ class Simulation:
                 "fired_in_degree", "fired_out_degree", "accepted_spikes", "pending_scanned",
                 "mask_words_stepped", "mask_words_written", "fired_records", "mask_words_read",
                 "bucket_entries", "records_read", "arrival_weight_writes", "cache_entries_written",
-                "arrival_records_read"]
+                "arrival_records_read", "post_weight_writes"]
         return dict(zip(keys, (int(x) for x in out)))
 
     def set_launch(self, waves_per_block=4, xcd_swizzle=0, graph_steps=0, stage_tables=0, fanout_waves=0,

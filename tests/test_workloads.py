@@ -57,11 +57,12 @@ def test_neuron_kernel_byte_model():
     # bytes per unit, as the docstrings' tables state them
     # (an active slot read from its record instead of the cache: 16 - 8)
     per_unit = {"active_slot_steps": 8, "records_read": 8, "bucket_entries": 8, "arrival_records_read": 16,
-                "arrival_weight_writes": 4, "cache_entries_written": 8, "fired": 12, "fired_in_degree": 20,
-                "pending_scanned": 4, "mask_words_read": 4, "mask_words_written": 4, "arrivals": 0}
+                "arrival_weight_writes": 4, "cache_entries_written": 8, "fired": 12, "fired_in_degree": 16,
+                "post_weight_writes": 4, "fired_records": 16, "pending_scanned": 4, "mask_words_read": 4,
+                "mask_words_written": 4, "arrivals": 0}
     zero = {k: 0 for k in per_unit}
     base, _ = bench.neuron_kernel_bytes(640, zero, zero, 10)
-    assert base == 20 * 640 + 48 * 10  # per neuron and per 64-neuron slice
+    assert base == 20 * 640 + 32 * 10  # per neuron and per 64-neuron slice
     for k, bpu in per_unit.items():
         c1 = dict(zero, **{k: 30})  # 3 units per step over 10 steps
         b, ev = bench.neuron_kernel_bytes(640, zero, c1, 10)
