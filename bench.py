@@ -361,6 +361,7 @@ def bench_served(args):
         res, el = SV.run_alone(job.spec)
         assert res == job.result, "job %d differs alone and in the mix" % job.idx
         seq_t.append(el)
+        print("[bench] job %d alone: %.2fs, identical" % (job.idx, el), file=sys.stderr, flush=True)
         feats.append([1.0, job.n_syn, job.n_syn * job.steps])
     seq = None
     if len(seq_t) >= 3:
