@@ -46,7 +46,7 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
       16 B per record read   at arrival (the target fired recently, or the
                              spike came through a pending list)
        4 B per weight        written by the arrival's pre-side STDP
-       8 B per cache entry   written (lanes whose active set changed)
+      16 B per cache record  written (lanes whose active set changed)
       16 B per in-synapse    post-side STDP of a fired neuron: record read
            of a fired neuron
        4 B per weight        written by post-side STDP (bits changed)
@@ -57,16 +57,16 @@ def neuron_kernel_bytes(n_neurons, c0, c1, steps):
     d = {k: (c1[k] - c0[k]) / steps for k in ("active_slot_steps", "arrivals", "fired", "fired_in_degree",
                                                "pending_scanned", "mask_words_read", "mask_words_written",
                                                "bucket_entries", "records_read", "arrival_weight_writes",
-                                               "cache_entries_written", "arrival_records_read", "fired_records",
+                                               "cache_records_written", "arrival_records_read", "fired_records",
                                                "post_weight_writes")}
     cached = d["active_slot_steps"] - d["records_read"]
     b = (20.0 * n_neurons + 32.0 * n_neurons / 64 + 4.0 * (d["mask_words_read"] + d["mask_words_written"])
          + 8.0 * cached + 16.0 * d["records_read"] + 8.0 * d["bucket_entries"] + 16.0 * d["arrival_records_read"]
-         + 4.0 * d["arrival_weight_writes"] + 8.0 * d["cache_entries_written"] + 16.0 * d["fired_in_degree"]
+         + 4.0 * d["arrival_weight_writes"] + 16.0 * d["cache_records_written"] + 16.0 * d["fired_in_degree"]
          + 4.0 * d["post_weight_writes"] + 16.0 * d["fired_records"] + 12.0 * d["fired"] + 4.0 * d["pending_scanned"])
     return b, {"A": d["active_slot_steps"], "A_records": d["records_read"], "R": d["arrivals"],
                "R_bucket": d["bucket_entries"], "R_records": d["arrival_records_read"],
-               "R_w": d["arrival_weight_writes"], "cache_w": d["cache_entries_written"], "F": d["fired"],
+               "R_w": d["arrival_weight_writes"], "cache_w": d["cache_records_written"], "F": d["fired"],
                "F_in": d["fired_in_degree"], "F_in_w": d["post_weight_writes"], "records": d["fired_records"],
                "pending": d["pending_scanned"], "mask_words": d["mask_words_read"],
                "mask_words_written": d["mask_words_written"]}

@@ -280,7 +280,7 @@ nc_status nc_reset_activities(nc_sim* sim, int32_t net_id);
  * summary bit flags), [13] arrival-bucket entries taken, [14] slot records
  * read by the charge (active slots past the active-slot cache), [15] weights
  * written at arrival (pre-side STDP that changed the weight), [16]
- * active-slot cache entries written, [17] slot records read at arrival (the
+ * active-slot cache records written (16 bytes each), [17] slot records read at arrival (the
  * target fired recently, or the spike came through a pending list), [18]
  * weights written by post-side STDP (those whose bits changed).
  * [0], [1] and [9] are kept by the host; the others are device counters that

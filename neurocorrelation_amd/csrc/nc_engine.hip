@@ -1807,8 +1807,10 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   std::vector<nc_slot> hs(r->nslots);
   NC_HIP(hipMemcpy(hs.data(), sim->slots + r->slot0, r->nslots * sizeof(nc_slot), hipMemcpyDeviceToHost));
   std::vector<uint32_t> masks(r->nmwords, 0u);
-  // the active-slot cache: each position's first NC_ACACHE active slots
-  std::vector<uint2> cache((size_t)NC_ACACHE * npos, make_uint2(0u, 0u));  // [position][entry]
+  // the active-slot cache: each position's first NC_ACACHE active slots,
+  // {depol 0, depol 1, depol 2, low bytes of their arrivals}
+  static_assert(NC_ACACHE == 3, "three depolarisations and their arrivals' low bytes per 16-byte record");
+  std::vector<uint4> cache((size_t)npos, make_uint4(0u, 0u, 0u, 0u));
   std::vector<uint8_t> ncache((size_t)npos, 0u);
   std::vector<std::vector<uint2>> pend(r->nq);
   std::vector<int32_t> edge_arr(r->S);
@@ -1833,7 +1835,12 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
         asum[pl] |= 1u << ((k >> 5) / G);
         uint32_t dbits;
         memcpy(&dbits, &x.depol, 4);
-        if (ncache[pl] < NC_ACACHE) cache[(size_t)pl * NC_ACACHE + ncache[pl]++] = make_uint2(dbits, (uint32_t)arr);
+        if (ncache[pl] < NC_ACACHE) {
+          uint4& c = cache[pl];
+          const uint32_t e = ncache[pl]++;
+          (e == 0 ? c.x : e == 1 ? c.y : c.z) = dbits;
+          c.w |= ((uint32_t)arr & 255u) << (8 * e);
+        }
       } else if (arr > s) {
         pend[q].push_back(make_uint2((uint32_t)r->slot0 + r->maps->slot_of_syn[syn], (uint32_t)arr));
       }
@@ -1863,7 +1870,6 @@ extern "C" nc_status nc_sim_restore(nc_sim* sim, int32_t net_id, const void* buf
   NC_HIP(hipMemcpyAsync(sim->asum + pos0, asum.data(), npos * 4, hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->slots + r->slot0, hs.data(), r->nslots * sizeof(nc_slot), hipMemcpyHostToDevice, sq));
   NC_HIP(hipMemcpyAsync(sim->amask + r->mask0, masks.data(), r->nmwords * 4, hipMemcpyHostToDevice, sq));
-  static_assert(sizeof(uint4) == NC_ACACHE * sizeof(uint2), "one 16-byte cache entry per position");
   NC_HIP(hipMemcpyAsync(sim->acache + pos0, cache.data(), npos * sizeof(uint4), hipMemcpyHostToDevice, sq));
   // each slice's spikes in flight into the pending list of their arrival
   // window, entries relative to the slice's first slot

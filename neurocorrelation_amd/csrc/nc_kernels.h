@@ -71,16 +71,16 @@
 #define NC_CHUNK (1u << NC_CHUNK_LOG)
 #define NC_MAX_IN_DEGREE 262140u  // slots of a slice stay within 2^24 (64 lanes x 32768 chunks of 8)
 #define NC_NCTR 19  // counter columns (device-side: 2-8, 10-18)
-// Each lane keeps (depolarisation, arrival) of its first NC_ACACHE active
-// slots, in slot order, in a per-position cache entry (16 bytes: the pairs
-// of position p at acache[p]): the charge reads those with one coalesced
-// load and only further active slots from their scattered slot records.  A
-// slot's depolarisation and arrival do not change while it is active (its
-// source cannot fire into it before the latch releases).  Two entries: at
-// 10^6 neurons and ~1.1 active slots per neuron and step, 88% of active
-// slots come from the cache, and four entries cost more cache traffic than
-// they save in record reads (a measured 68 against 61 us per step).
-#define NC_ACACHE 2
+// Each lane keeps the depolarisation and arrival of its first NC_ACACHE
+// active slots, in slot order, in one 16-byte record per position
+// (acache[p] = {depol 0, depol 1, depol 2, arrival 0 & 255 | (arrival 1 &
+// 255) << 8 | (arrival 2 & 255) << 16}): the charge reads those with one
+// coalesced load and only further active slots from their scattered slot
+// records, each a dependent read of several microseconds under load.  An
+// active slot's arrival is one of the last R < 250 steps, so its low byte
+// gives it back.  A slot's depolarisation and arrival do not change while it
+// is active (its source cannot fire into it before the latch releases).
+#define NC_ACACHE 3
 // Slot records fetched together by the charge for active slots past the
 // cache (one: fewer registers, a wave more per SIMD; 57 against 59 us per step
 // of k_neuron at 10^6 neurons).
@@ -216,16 +216,18 @@ __device__ __forceinline__ uint2 nc_pick(const uint2 (&c)[NC_ACACHE], uint32_t i
   return v;
 }
 
-// Inserts in-slot k with value v into the ascending list (nk, nv) of n
-// entries, keeping the NC_ACACHE smallest slots.  Slot numbers are distinct;
-// entries past n hold nothing, so whatever the shifts leave there is unused.
-__device__ __forceinline__ void nc_push(uint32_t (&nk)[NC_ACACHE], uint2 (&nv)[NC_ACACHE], uint32_t& n, uint32_t k,
-                                        uint2 v) {
+// Inserts in-slot k (below 2^24) arriving at step arr, with depolarisation
+// bits v, into the ascending list (nk, nv) of n entries, keeping the
+// NC_ACACHE smallest slots; nk holds slot | (arr & 255) << 24.  Slot numbers
+// are distinct; entries past n hold nothing, so whatever the shifts leave
+// there is unused.
+__device__ __forceinline__ void nc_push(uint32_t (&nk)[NC_ACACHE], uint32_t (&nv)[NC_ACACHE], uint32_t& n, uint32_t k,
+                                        uint32_t arr, uint32_t v) {
+  k |= arr << 24;
 #pragma unroll
   for (int e = 0; e < NC_ACACHE; ++e) {
-    const bool here = (uint32_t)e >= n || k < nk[e];
-    const uint32_t tk = nk[e];
-    const uint2 tv = nv[e];
+    const bool here = (uint32_t)e >= n || (k & 0xFFFFFFu) < (nk[e] & 0xFFFFFFu);
+    const uint32_t tk = nk[e], tv = nv[e];
     nk[e] = here ? k : tk;
     nv[e] = here ? v : tv;
     k = here ? tk : k;
@@ -436,8 +438,11 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   // The bucket's entries are taken up after the fire.
   const uint32_t b_take = b_n < NC_BCAP ? b_n : NC_BCAP;
   // The lane's cache of (depolarisation, arrival) of its first NC_ACACHE
-  // active slots in slot order.
-  const uint2 ac[NC_ACACHE] = {make_uint2(cache.x, cache.y), make_uint2(cache.z, cache.w)};
+  // active slots in slot order (an arrival from its low byte: one of the
+  // last R steps).
+  const uint2 ac[NC_ACACHE] = {make_uint2(cache.x, (uint32_t)s - (((uint32_t)s - cache.w) & 255u)),
+                               make_uint2(cache.y, (uint32_t)s - (((uint32_t)s - (cache.w >> 8)) & 255u)),
+                               make_uint2(cache.z, (uint32_t)s - (((uint32_t)s - (cache.w >> 16)) & 255u))};
 
   // charge_insynapses (src/NeuCor.cpp:688-700): the active slots (spikes
   // that arrived at s-R .. s-1) in ascending source order.  A lane walks its
@@ -445,8 +450,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   // arrival come from the cache while i < NC_ACACHE, else from its slot
   // record.  The latch is released at o = R (:697).  The kept slots, then the
   // arrivals of this step, form the next cache.
-  uint32_t nk[NC_ACACHE];  // the next cache: slot numbers and (depol, arrival), ascending
-  uint2 nv[NC_ACACHE] = {make_uint2(0u, 0u), make_uint2(0u, 0u)};
+  // the next cache, ascending: slot | (arrival & 255) << 24, depolarisation bits
+  uint32_t nk[NC_ACACHE] = {0u, 0u, 0u}, nv[NC_ACACHE] = {0u, 0u, 0u};
   uint32_t ncnt = 0, idx = 0;
   bool cache_dirty = false;
   uint32_t kw[4] = {0u, 0u, 0u, 0u};  // kept bits of mask words 0-3
@@ -504,7 +509,7 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
           else khi &= ~(1ull << (pp[e] - 64u));
           cache_dirty = true;
         } else {
-          nc_push(nk, nv, ncnt, 32u * w0 + pp[e], dv[e]);
+          nc_push(nk, nv, ncnt, 32u * w0 + pp[e], dv[e].y, dv[e].x);
         }
       }
       idx += cnt;
@@ -710,7 +715,7 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
             ++n_wset;
           }
         }
-        nc_push(nk, nv, ncnt, k, make_uint2(dk, (uint32_t)s));
+        nc_push(nk, nv, ncnt, k, (uint32_t)s, dk);
         cache_dirty = true;
         ++n_arr;
       }
@@ -736,8 +741,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   if (cache_dirty) {
     // one 16-byte store; entries past ncnt are never read (the lane has
     // ncnt active slots)
-    a.acache[pos] = make_uint4(nv[0].x, nv[0].y, nv[1].x, nv[1].y);
-    n_cw += NC_ACACHE;
+    a.acache[pos] = make_uint4(nv[0], nv[1], nv[2], (nk[0] >> 24) | ((nk[1] >> 24) << 8) | ((nk[2] >> 24) << 16));
+    ++n_cw;
   }
   if (live && sm_keep != sm) a.asum[pos] = sm_keep;
   // The neuron's own state is written last (nothing in this kernel reads it

@@ -222,7 +222,7 @@ class Simulation:
         keys = ["neuron_steps", "synapse_steps", "active_slot_steps", "arrivals", "fired",
                 "fired_in_degree", "fired_out_degree", "accepted_spikes", "pending_scanned",
                 "mask_words_stepped", "mask_words_written", "fired_records", "mask_words_read",
-                "bucket_entries", "records_read", "arrival_weight_writes", "cache_entries_written",
+                "bucket_entries", "records_read", "arrival_weight_writes", "cache_records_written",
                 "arrival_records_read", "post_weight_writes"]
         return dict(zip(keys, (int(x) for x in out)))
 

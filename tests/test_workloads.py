@@ -57,7 +57,7 @@ def test_neuron_kernel_byte_model():
     # bytes per unit, as the docstrings' tables state them
     # (an active slot read from its record instead of the cache: 16 - 8)
     per_unit = {"active_slot_steps": 8, "records_read": 8, "bucket_entries": 8, "arrival_records_read": 16,
-                "arrival_weight_writes": 4, "cache_entries_written": 8, "fired": 12, "fired_in_degree": 16,
+                "arrival_weight_writes": 4, "cache_records_written": 16, "fired": 12, "fired_in_degree": 16,
                 "post_weight_writes": 4, "fired_records": 16, "pending_scanned": 4, "mask_words_read": 4,
                 "mask_words_written": 4, "arrivals": 0}
     zero = {k: 0 for k in per_unit}
