@@ -61,9 +61,13 @@
 // lanes' edges fill at the active networks' mean out-degree), so that a burst
 // of fired lanes in one slice is shared by up to NC_FSPLIT fan-out waves;
 // with fewer waves than slices every wave has records enough and n = 1.  A
-// record is {mask lo, mask hi, step, slice | i << 24 | (n - 1) << 28}.
+// record is {mask lo, mask hi, step, slice | i << 24 | (n - 1) << 28}; when
+// one lane fired (most fired slices at 1% activity) it is {first out-edge,
+// out-degree, step, slice | NC_FSINGLE | i << 24 | (n - 1) << 28}, which
+// spares fan-out the out-row load and the prefix sum.
 #define NC_FSPLIT 16u
-#define NC_FSLICE_BITS 24
+#define NC_FSLICE_BITS 23
+#define NC_FSINGLE (1u << 23)
 #define NC_FCNT_STRIDE 32  // words between two counters (one cache line each)
 // In-slots are stored chunk-major: NC_CHUNK consecutive slots of a lane (one
 // 128-byte line of 16-byte records), then the next lane's.
@@ -580,13 +584,21 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
       // the fired lanes' out-edges, in 64-edge groups, shared by n records
       // (written by one lane: a record per lane would hold registers at the
       // kernel's tightest point)
-      const uint32_t n_frec =
-          a.fsplit_od ? min(NC_FSPLIT, ((uint32_t)__builtin_popcountll(fm) * a.fsplit_od + 63u) >> 6) : 1u;
+      const uint32_t nfl = (uint32_t)__builtin_popcountll(fm);
+      const uint32_t n_frec = a.fsplit_od ? min(NC_FSPLIT, (nfl * a.fsplit_od + 63u) >> 6) : 1u;
       const uint32_t b = wave % NC_FLISTS;  // waves running together use different lists
+      uint32_t rx = (uint32_t)fm, ry = (uint32_t)(fm >> 32), tag = q;
+      if (nfl == 1u) {
+        // one fired lane: its out-edge range, loaded beside the list atomic
+        const uint2 orow = a.out_row[q * 64u + (uint32_t)__builtin_ctzll(fm)];
+        rx = orow.x;
+        ry = orow.y;
+        tag |= NC_FSINGLE;
+      }
       const uint32_t at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], n_frec);
       for (uint32_t i = 0; i < n_frec; ++i)
-        a.fired[b * a.fcap + at + i] = make_uint4((uint32_t)fm, (uint32_t)(fm >> 32), (uint32_t)s,
-                                                  q | (i << NC_FSLICE_BITS) | ((n_frec - 1u) << (NC_FSLICE_BITS + 4)));
+        a.fired[b * a.fcap + at + i] =
+            make_uint4(rx, ry, (uint32_t)s, tag | (i << 24) | ((n_frec - 1u) << 28));
       if (a.ctr) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 11], (unsigned long long)n_frec);
     }
   }
@@ -878,7 +890,20 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
       const int32_t s = __shfl((int)fl.z, (int)fsrc);
       const uint32_t fw = (uint32_t)__shfl((int)fl.w, (int)fsrc);
       const uint32_t fq = fw & ((1u << NC_FSLICE_BITS) - 1u);
-      const uint32_t item = (fw >> NC_FSLICE_BITS) & 15u, n = (fw >> (NC_FSLICE_BITS + 4)) + 1u;
+      const uint32_t item = (fw >> 24) & 15u, n = (fw >> 28) + 1u;
+      if (fw & NC_FSINGLE) {
+        // one fired lane: its out-edges are [fx, fx + fy); this record's
+        // share is groups item, item + n, ...
+        if (item == 0u) n_out += fy;
+        for (uint32_t base = 64u * item; base < fy; base += 64u * n) {
+          const uint32_t i = base + lane;
+          if (i < fy) {
+            const nc_edge ed = a.edges[fx + i];
+            nc_fire_edge(a, fx + i, ed, s, n_acc);
+          }
+        }
+        continue;
+      }
       const unsigned long long m = ((unsigned long long)fy << 32) | fx;
       // The out-lists of all fired lanes, flattened: lane-parallel row loads,
       // a wave prefix sum of out-degrees, then this wave's 64-edge groups.
