@@ -682,6 +682,25 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     // entry (slot - first slot of the slice) -> the lane that owns the slot
     const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
     const uint32_t kk = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
+    // The weight now, the sign class and the latched depolarisation of each
+    // entry, found by the lane that holds the entry, all in one round: the
+    // entry's own while nothing can have changed the weight since the fire
+    // (the target has not fired since, and no host write came after it),
+    // else the slot record's (read only when the weight is used: pre-side
+    // STDP, or a pending-list spike).
+    const int32_t s_fire = s - (int32_t)((info & 63u) + 1u);
+    const int32_t lf_t = __shfl(lf, (int)tl);
+    const float tr_tt = __int_as_float(__shfl(__float_as_int(tr_t), (int)tl));
+    const bool fresh = known && lf_t <= s_fire && s_fire >= nd.whost;
+    uint32_t w_now = wbits, kinh = kk | ((info >> 7) << 31);
+    uint32_t dkv = __float_as_uint(a.depol_k * __uint_as_float(wbits));  // Synapse::fire (:730-732), as fan-out made it
+    if (ev && !fresh && (tr_tt != 0.0f || !known)) {
+      const nc_slot r = a.slots[nc_slot_of(rec.slot_off, tl, kk)];
+      ++n_arec;
+      w_now = __float_as_uint(r.w);
+      kinh = kk | (((uint32_t)r.arr_inh & 1u) << 31);
+      if (!known) dkv = __float_as_uint(r.depol);
+    }
     unsigned long long mine = __ballot(ev);
 #pragma unroll
     for (int bb = 0; bb < 6; ++bb) {
@@ -690,10 +709,11 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     }
     while (__ballot(mine != 0ull)) {
       const uint32_t src = mine ? (uint32_t)__builtin_ctzll(mine) : lane;
-      const uint32_t k = (uint32_t)__shfl((int)kk, (int)src);
-      const uint32_t inf = (uint32_t)__shfl((int)info, (int)src);
-      const uint32_t wb = (uint32_t)__shfl((int)wbits, (int)src);
+      const uint32_t ki = (uint32_t)__shfl((int)kinh, (int)src);
+      const uint32_t wb = (uint32_t)__shfl((int)w_now, (int)src);
+      const uint32_t dk = (uint32_t)__shfl((int)dkv, (int)src);
       if (mine) {
+        const uint32_t k = ki & 0x7FFFFFFFu;
         mine &= mine - 1;
         const uint32_t w = k >> 5, bit = 1u << (k & 31u);
         if (w < 4u) {
@@ -707,22 +727,9 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
         }
         sm_keep |= 1u << (G == 1u ? w : w / G);
         const uint32_t j = nc_slot_of(rec.slot_off, lane, k);
-        const int32_t s_fire = s - (int32_t)((inf & 63u) + 1u);
-        const bool fresh = known && lf <= s_fire && s_fire >= nd.whost;
-        // the weight now and its sign class: the entry's, or the record's
-        float w_now = __uint_as_float(wb);
-        uint32_t inh = inf >> 7;
-        uint32_t dk = __float_as_uint(a.depol_k * w_now);  // Synapse::fire (:730-732), as fan-out made it
-        if (!fresh && (tr_t != 0.0f || !known)) {
-          const nc_slot r = a.slots[j];
-          ++n_arec;
-          w_now = r.w;
-          inh = (uint32_t)r.arr_inh & 1u;
-          if (!known) dk = __float_as_uint(r.depol);
-        }
         if (tr_t != 0.0f) {
-          const float wn = nc_stdp(w_now, inh, 0.0f, tr_t, pre_f, post_f, lr);
-          if (__float_as_uint(wn) != __float_as_uint(w_now)) {
+          const float wn = nc_stdp(__uint_as_float(wb), ki >> 31, 0.0f, tr_t, pre_f, post_f, lr);
+          if (__float_as_uint(wn) != wb) {
             a.slots[j].w = wn;
             ++n_wset;
           }
