@@ -426,7 +426,8 @@ def main():
     ap.add_argument("--nets", type=int, default=4096, help="batched-* workloads: number of networks")
     ap.add_argument("--sample", type=int, default=12, help="batched-*: networks rerun on the plain path")
     ap.add_argument("--tune-steps", type=int, default=16,
-                    help="batched-*: steps per autotuner trial (0: no autotuning, --waves-per-block launch)")
+                    help="steps per autotuner trial before the timed steps (0: no autotuning; batched-*: the "
+                         "--waves-per-block launch, plain: the engine's default launch or --launch)")
     ap.add_argument("--waves-per-block", type=int, default=4, help="batched-*: launch shape when not autotuning")
     ap.add_argument("--launch", default="",
                     help='plain workloads: launch shape as JSON, e.g. {"persistent": 1, "waves_per_block": 8}')
@@ -491,6 +492,22 @@ def main():
 
     run(sim, args.warmup)
     sim.synchronize()
+    tuning = None
+    if not args.launch and args.tune_steps:
+        # The launch shape autotuned on the live run, every candidate timed
+        # from the same saved state (no simulated step is taken): slices per
+        # workgroup x the XCD-ordered remap x trace tables in LDS, ordinary
+        # launches (the kernel events ride on them).
+        cands = [dict(waves_per_block=w, xcd_swizzle=x & 1, stage_tables=x >> 1) for w in (1, 2, 4, 8)
+                 for x in range(4)]
+        t_tune = time.perf_counter()
+        chosen, ms = sim.tune_list(cands, steps_per_trial=args.tune_steps, rounds=3)
+        launch = {k: chosen[k] for k in ("waves_per_block", "xcd_swizzle", "stage_tables")}
+        tuning = {"s": time.perf_counter() - t_tune, "steps_per_trial": args.tune_steps, "rounds": 3,
+                  "table": [dict(c, median_ms=float(m)) for c, m in zip(cands, ms)]}
+        print("[bench] tuned launch %s (%.2fs; default shape %.3f ms, chosen %.3f ms per trial)" %
+              (launch, tuning["s"], float(ms[cands.index(dict(waves_per_block=4, xcd_swizzle=0, stage_tables=0))]),
+               float(ms.min())), file=sys.stderr, flush=True)
     every = TIMING_EVERY if args.steps >= 200 else 1
     if not args.no_kernel_timing:
         # start/stop events carried by the neuron-step and fan-out dispatches
@@ -531,6 +548,7 @@ def main():
     else:
         sim = nc.Simulation.for_network(net)
         cid = sim.add(net, params, drive)
+        sim.set_launch(**launch)
         run(sim, args.warmup)
         sim.set_counting(True)
         c0 = sim.counters()
@@ -608,6 +626,7 @@ def main():
                      "avg_launch_us": K["avg_launch_us"], "launches": K["launches"]},
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "launch_tuning": tuning,
     }
     print(json.dumps(out), flush=True)
 

@@ -1249,7 +1249,11 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   // slice's out-edges are split over several records only when that grid has
   // a wave per active slice or more (else every wave has records enough):
   // about one record per 64 edges at the active networks' mean out-degree.
-  a.fcap = (grid * wpb + NC_FLISTS - 1) / NC_FLISTS * NC_FSPLIT;
+  // With XCD lists (the XCD-ordered remap, ordinary launches), list (x, j)
+  // takes the k_neuron waves of workgroups b = x mod 8 whose rank on the XCD
+  // is j mod 8.
+  a.xcd_lists = (sim->cfg.xcd_swizzle && !sim->cfg.persistent) ? 1u : 0u;
+  a.fcap = (a.xcd_lists ? ((grid + 7) / 8 * wpb + 7) / 8 : (grid * wpb + NC_FLISTS - 1) / NC_FLISTS) * NC_FSPLIT;
   uint64_t syn_active = 0, n_active = 0;
   for (size_t i = 0; i < sim->nets.size(); ++i)
     if (sim->nets[i].live && nds[i].remaining > 0) {
@@ -1264,7 +1268,9 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
       (uint64_t)grid * wpb + (split ? (uint64_t)grid * wpb + syn_active / 64 : 0u);
   const uint32_t fan_waves =
       (uint32_t)std::min<uint64_t>(wave_cap, (rec_bound + NC_FLISTS - 1) / NC_FLISTS * NC_FLISTS);
-  const uint32_t fan_grid = std::max<uint32_t>((fan_waves + wpb - 1) / wpb, (NC_FLISTS + wpb - 1) / wpb);
+  uint32_t fan_grid = std::max<uint32_t>((fan_waves + wpb - 1) / wpb, (NC_FLISTS + wpb - 1) / wpb);
+  // XCD lists: the same number of waves on every XCD, 8 per list round
+  if (a.xcd_lists) fan_grid = (fan_grid + NC_FLISTS / wpb - 1) / (NC_FLISTS / wpb) * (NC_FLISTS / wpb);
   const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
   const void* fn_neuron = nullptr;
   const void* fn_fanout = nullptr;

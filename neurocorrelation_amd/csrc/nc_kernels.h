@@ -174,6 +174,8 @@ struct nc_step_args {
   int32_t chunk_i;            // step of the launch inside its chunk
   const uint32_t* chunk_off;  // device: steps of this call before the chunk
   uint32_t xcd_swizzle;       // blocks of one XCD take one contiguous range of slices
+  uint32_t xcd_lists;         // fired lists by XCD: a wave of workgroup b uses list (b % 8) * 8 + its
+                              // rank among the XCD's waves % 8, in k_neuron and k_fanout alike
   const uint2* out_row;     // per position: first out-edge, out-degree
   nc_edge* edges;           // out-edges, rows in ascending target id
   uint32_t* pend0;         // pending list 0 of every slice, at its slot range
@@ -586,7 +588,10 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
       // kernel's tightest point)
       const uint32_t nfl = (uint32_t)__builtin_popcountll(fm);
       const uint32_t n_frec = a.fsplit_od ? min(NC_FSPLIT, (nfl * a.fsplit_od + 63u) >> 6) : 1u;
-      const uint32_t b = wave % NC_FLISTS;  // waves running together use different lists
+      // waves running together use different lists; with XCD lists, those of
+      // the workgroup's own XCD, which its fan-out waves read
+      const uint32_t b = a.xcd_lists ? (blockIdx.x & 7u) * 8u + ((((blockIdx.x >> 3) * WPB) + (threadIdx.x >> 6)) & 7u)
+                                     : wave % NC_FLISTS;
       uint32_t rx = (uint32_t)fm, ry = (uint32_t)(fm >> 32), tag = q;
       if (nfl == 1u) {
         // one fired lane: its out-edge range, loaded beside the list atomic
@@ -874,14 +879,12 @@ __device__ __forceinline__ void nc_fire_edge(const nc_step_args& a, uint32_t e, 
   ++n_acc;
 }
 
-__device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t blk, uint32_t wave, uint32_t nwaves,
-                                               int32_t ci) {
+// The wave takes list lst, entries first, first + per, ... (64 per load), per
+// = the waves per list.
+__device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t blk, uint32_t lst, uint32_t first,
+                                               uint32_t per, int32_t ci) {
   const uint32_t lane = threadIdx.x & 63;
   uint32_t n_acc = 0, n_out = 0;
-  // Wave w takes list w % NC_FLISTS, entries first, first + per, ... (64 per
-  // load), per = the waves per list.
-  const uint32_t per = nwaves / NC_FLISTS;
-  const uint32_t lst = wave % NC_FLISTS, first = wave / NC_FLISTS;
   const uint32_t nf = first < per ? __builtin_amdgcn_readfirstlane(
                                         a.fired_cnt[((((uint32_t)ci & 1u) * NC_FLISTS) + lst) * NC_FCNT_STRIDE])
                                   : 0u;
@@ -957,8 +960,20 @@ __device__ __forceinline__ void nc_fanout_wave(const nc_step_args& a, uint32_t b
 template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_fanout(nc_step_args a) {
   const uint32_t blk = nc_block_id(a.xcd_swizzle);
-  nc_fanout_wave(a, blk % a.ctr_rows, blk * WPB + (threadIdx.x >> 6), gridDim.x * WPB,
-                 (int32_t)*a.chunk_off + a.chunk_i);
+  const uint32_t per = gridDim.x * WPB / NC_FLISTS;
+  uint32_t lst, first;
+  if (a.xcd_lists) {
+    // the lists of this workgroup's XCD (the grid holds the same number of
+    // waves on every XCD, a multiple of 8)
+    const uint32_t wx = (blockIdx.x >> 3) * WPB + (threadIdx.x >> 6);
+    lst = (blockIdx.x & 7u) * 8u + (wx & 7u);
+    first = wx >> 3;
+  } else {
+    const uint32_t wave = blk * WPB + (threadIdx.x >> 6);
+    lst = wave % NC_FLISTS;
+    first = wave / NC_FLISTS;
+  }
+  nc_fanout_wave(a, blk % a.ctr_rows, lst, first, per, (int32_t)*a.chunk_off + a.chunk_i);
 }
 
 // Grid-wide barrier of the persistent step kernel: each workgroup adds one
@@ -1006,5 +1021,5 @@ __global__ __launch_bounds__(64 * WPB) void k_step(nc_step_args a, uint32_t* bar
   // every wave of the grid runs the same number of slice passes
   for (uint32_t b = 0; b < a.n_active_slices; b += nwaves) nc_neuron_slice<WPB>(a, blk, b + w0, ci, tpre, tpost);
   nc_grid_barrier(bar, ((uint32_t)ci + 1u) * gridDim.x, bar + 32);
-  nc_fanout_wave(a, blk, w0, nwaves, ci);
+  nc_fanout_wave(a, blk, w0 % NC_FLISTS, w0 / NC_FLISTS, nwaves / NC_FLISTS, ci);
 }
