@@ -493,11 +493,15 @@ def main():
     run(sim, args.warmup)
     sim.synchronize()
     tuning = None
-    if not args.launch and args.tune_steps:
-        # The launch shape autotuned on the live run, every candidate timed
-        # from the same saved state (no simulated step is taken): slices per
-        # workgroup x the XCD-ordered remap x trace tables in LDS, ordinary
-        # launches (the kernel events ride on them).
+    if not args.launch and args.tune_steps and args.steps >= 1000:
+        # The launch shape autotuned on the live run after the warm-up steps,
+        # every candidate timed from the same saved state (no simulated step
+        # is taken): slices per workgroup x the XCD-ordered remap x trace
+        # tables in LDS, ordinary launches (the kernel events ride on them).
+        # The tuner's 6 GB of state copies slow the next millisecond or two of
+        # steps (measured: 20 timed steps right after it at 119-153 us per
+        # step, against 82-84 us untuned): only runs of 1000 steps or more are
+        # tuned.
         cands = [dict(waves_per_block=w, xcd_swizzle=x & 1, stage_tables=x >> 1) for w in (1, 2, 4, 8)
                  for x in range(4)]
         t_tune = time.perf_counter()
