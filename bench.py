@@ -24,7 +24,14 @@ sys.path.insert(0, ROOT)
 
 METRIC = "synapse updates/s (% HBM roofline); served sims/s + p50/p99 vs plain"
 HBM_PEAK_GBS = 8000.0
-TIMING_EVERY = 10  # kernel events on every 10th step's launches from 200 timed steps on, every step below
+TIMING_EVERY = 10  # kernel events on every 10th step's launches at most; at least 5 timed launches per kernel
+# Launch shapes that the live tuner chose for a plain workload on MI355X, used
+# by runs too short to tune (under 1000 timed steps). plain-1m: 1 slice per
+# workgroup with the XCD-ordered remap, 1.437 ms per 16-step trial after the
+# warm-up against 1.488 ms for the engine's default (4 slices per workgroup, no
+# remap); 20 timed steps from step 6, three runs each: 73.1 against 75.6 us per
+# step.
+TUNED_LAUNCH = {"plain-1m": {"waves_per_block": 1, "xcd_swizzle": 1}}
 
 
 def neuron_kernel_bytes(n_neurons, c0, c1, steps):
@@ -477,7 +484,12 @@ def main():
     N, S = info["n"], info["n_syn"]
     sim = nc.Simulation.for_network(net)
     nid = sim.add(net, params, drive)
-    launch = json.loads(args.launch) if args.launch else {}
+    if args.launch:
+        launch, launch_source = json.loads(args.launch), "argument"
+    elif args.workload in TUNED_LAUNCH:
+        launch, launch_source = dict(TUNED_LAUNCH[args.workload]), "table of tuned shapes"
+    else:
+        launch, launch_source = {}, "engine default"
     sim.set_launch(**launch)
     setup_s = time.perf_counter() - t0
     print("[bench] %s: %d neurons, %d synapses, setup %.1fs, launch %s" % (args.workload, N, S, setup_s,
@@ -500,23 +512,25 @@ def main():
         # tables in LDS, ordinary launches (the kernel events ride on them).
         # The tuner's 6 GB of state copies slow the next millisecond or two of
         # steps (measured: 20 timed steps right after it at 119-153 us per
-        # step, against 82-84 us untuned): only runs of 1000 steps or more are
-        # tuned.
+        # step, against 82-84 us untuned, both with kernel events on every step):
+        # only runs of 1000 steps or more are tuned.
         cands = [dict(waves_per_block=w, xcd_swizzle=x & 1, stage_tables=x >> 1) for w in (1, 2, 4, 8)
                  for x in range(4)]
         t_tune = time.perf_counter()
         chosen, ms = sim.tune_list(cands, steps_per_trial=args.tune_steps, rounds=3)
         launch = {k: chosen[k] for k in ("waves_per_block", "xcd_swizzle", "stage_tables")}
+        launch_source = "tuned on the live run"
         tuning = {"s": time.perf_counter() - t_tune, "steps_per_trial": args.tune_steps, "rounds": 3,
                   "table": [dict(c, median_ms=float(m)) for c, m in zip(cands, ms)]}
         print("[bench] tuned launch %s (%.2fs; default shape %.3f ms, chosen %.3f ms per trial)" %
               (launch, tuning["s"], float(ms[cands.index(dict(waves_per_block=4, xcd_swizzle=0, stage_tables=0))]),
                float(ms.min())), file=sys.stderr, flush=True)
-    every = TIMING_EVERY if args.steps >= 200 else 1
+    every = min(TIMING_EVERY, max(1, args.steps // 5))
     if not args.no_kernel_timing:
         # start/stop events carried by the neuron-step and fan-out dispatches
-        # (of every TIMING_EVERY-th step on long runs: events on every launch
-        # slow the queue a little)
+        # of every `every`-th step: each step that carries them takes about
+        # 14 us longer at plain-1m (20 timed steps with events on every step:
+        # 85.6 us per step, 71.9 without), so they are spread thin
         sim.set_kernel_timing((1 << 1) | (1 << 3), every)
     t_start = time.perf_counter()
     run(sim, args.steps)
@@ -630,6 +644,7 @@ def main():
                      "avg_launch_us": K["avg_launch_us"], "launches": K["launches"]},
         "kernels": kernels,
         "cpu_baseline": cpu,
+        "launch_source": launch_source,
         "launch_tuning": tuning,
     }
     print(json.dumps(out), flush=True)
