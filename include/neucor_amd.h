@@ -331,9 +331,9 @@ nc_status nc_set_counting(nc_sim* sim, int enable);
  * slices; graph_steps: steps per captured hipGraph chunk (0 = direct
  * launches, at most 1024; a chunk is replayed while the set of networks is
  * unchanged);
- * stage_tables: copy the STDP trace tables into LDS per workgroup (pays when
- * many synapses see STDP per step) instead of reading them through the
- * caches; fanout_waves: the most waves of the fan-out kernel, which share
+ * stage_tables: copy the synapse-trace table of post-side STDP into LDS per
+ * workgroup (pays when many synapses see STDP per step) instead of reading
+ * it through the caches; fanout_waves: the most waves of the fan-out kernel, which share
  * the step's records of slices with fired neurons (below the cap, one
  * record per wave; never more waves than records can need); persistent:
  * the whole step (neuron phase, a grid-wide barrier, fan-out) in one launch

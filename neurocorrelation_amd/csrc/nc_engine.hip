@@ -537,7 +537,7 @@ nc_step_args base_args(nc_sim* sim) {
   a.w_ap = sim->w_ap;
   a.len_pre = (int32_t)sim->T.t_pre.size();
   a.len_post = (int32_t)sim->T.t_post.size();
-  a.tabs_in_lds = sim->cfg.stage_tables && (uint64_t)(a.len_pre + a.len_post) * 4 <= NC_LDS_TAB_BYTES ? 1 : 0;
+  a.tabs_in_lds = sim->cfg.stage_tables && (uint64_t)a.len_pre * 4 <= NC_LDS_TAB_BYTES ? 1 : 0;
   a.chunk_off = sim->d_chunk_off;
   a.xcd_swizzle = sim->cfg.xcd_swizzle;
   for (int k = 0; k < NC_WAP_K; ++k)
@@ -1271,7 +1271,7 @@ extern "C" nc_status nc_sim_step(nc_sim* sim, uint32_t n_steps) try {
   uint32_t fan_grid = std::max<uint32_t>((fan_waves + wpb - 1) / wpb, (NC_FLISTS + wpb - 1) / wpb);
   // XCD lists: the same number of waves on every XCD, 8 per list round
   if (a.xcd_lists) fan_grid = (fan_grid + NC_FLISTS / wpb - 1) / (NC_FLISTS / wpb) * (NC_FLISTS / wpb);
-  const uint32_t lds = a.tabs_in_lds ? (uint32_t)(a.len_pre + a.len_post) * 4 : 0u;
+  const uint32_t lds = a.tabs_in_lds ? (uint32_t)a.len_pre * 4 : 0u;
   const void* fn_neuron = nullptr;
   const void* fn_fanout = nullptr;
   switch (wpb) {

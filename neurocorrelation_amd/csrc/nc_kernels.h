@@ -32,7 +32,7 @@
 #define NC_DEG_INPUT 0x01000000u
 #define NC_DEG_VALID 0x02000000u
 #define NC_SLICE_NONE 0xFFFFFFFFu
-#define NC_LDS_TAB_BYTES 32768  // trace tables up to this size are staged in LDS
+#define NC_LDS_TAB_BYTES 32768  // a pre-trace table up to this size is staged in LDS
 #define NC_WAP_K 8       // AP-table entries passed as kernel arguments
 // Spikes in flight to a slice sit in one of two pending lists by arrival
 // window: list (arrival >> NC_PEND_LOG_W) & 1, each at the slice's slot range
@@ -166,7 +166,7 @@ struct nc_step_args {
   const float* t_post;
   const float* w_ap;
   int32_t len_pre, len_post;
-  int32_t tabs_in_lds;     // both trace tables are staged in (dynamic) LDS
+  int32_t tabs_in_lds;     // the pre-trace table is staged in (dynamic) LDS
   float w_ap_k[NC_WAP_K];  // the AP table when it has <= NC_WAP_K entries
   float dt, P, depol_k;
   double c0, e;
@@ -390,7 +390,7 @@ __global__ void k_drive(nc_step_args a) {
 // blk is the workgroup's counter row).
 template <int WPB>
 __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t blk, uint32_t wave, int32_t ci,
-                                                const float* tpre, const float* tpost) {
+                                                const float* tpre) {
   const uint32_t lane = threadIdx.x & 63;
   const bool wave_in = wave < a.n_active_slices;
   const uint32_t widx = wave_in ? wave : 0u;
@@ -426,6 +426,14 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
                                  ((nw.x & NC_DEG_INPUT) ? (nw.y >> 5) : 0u)];
   const uint32_t plist = ((uint32_t)s >> NC_PEND_LOG_W) & 1u;
   uint32_t pend_n = plist ? pc.y : pc.x;
+  // The target trace at s of a lane that does not fire now (its last fire
+  // stays lf), read with the mask words instead of after the fire: from a
+  // clamped place (the table holds at least its entry 0) of the table in
+  // global memory, whatever the launch shape, with no branch around the load
+  // and its use after the fire. A lane that fires now has trace 0: entry 0
+  // of every trace table, where powf(decay, 0) == 1 counts as no trace.
+  const int32_t dlf0 = s - lf;
+  const float tr_tab = a.t_post[min(max(dlf0, 0), a.len_post - 1)];
   if (!live) {
     nw = make_uint2(0u, 0u);
     pend_n = 0u;
@@ -559,6 +567,7 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   // charge_thresholdCheck (src/NeuCor.cpp:682-686) + input fires (:326-331).
   const bool refr_ok = (s - lf) > a.d_ref;
   const bool fire = valid && ((((NC_THRESHOLD < V) || f_bg) && refr_ok) || f_in);
+  const bool tr_in = (uint32_t)(s - lf) < (uint32_t)a.len_post;  // the last fire before this step's
   if (fire) lf = s;
   // AP waveform (src/NeuCor.cpp:706-714) from the host table.
   const int32_t dlf = s - lf;
@@ -625,26 +634,40 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
         const bool two = k + 64 < dl;
         const uint32_t j0 = nc_slot_of(rec.slot_off, l, k);
         const uint32_t j1 = nc_slot_of(rec.slot_off, l, k + 64);
-        const nc_slot s0 = a.slots[j0];
-        nc_slot s1 = s0;
-        if (two) s1 = a.slots[j1];
+        // both records in one round (the first again when there is no second;
+        // the empty asm keeps the compiler from waiting for the first before
+        // it issues the second)
+        nc_slot s0 = a.slots[j0];
+        nc_slot s1 = a.slots[two ? j1 : j0];
+        asm volatile("" : "+v"(s0.w), "+v"(s0.last_arr), "+v"(s0.arr_inh), "+v"(s1.w), "+v"(s1.last_arr),
+                     "+v"(s1.arr_inh));
         const int32_t a0 = s0.arr_inh >> 1, a1 = s1.arr_inh >> 1;
         // the record's arrival if it is a step in [1, s-1], else last_arr
         const int32_t la0 = (uint32_t)(a0 - 1) < (uint32_t)(s - 1) ? a0 : s0.last_arr;
         const int32_t la1 = (uint32_t)(a1 - 1) < (uint32_t)(s - 1) ? a1 : s1.last_arr;
+        // Both trace reads in one round, before either store (a read after a
+        // store to memory it may alias waits for that store): from clamped
+        // places (the table holds at least its entry 0), with no branch
+        // around them; the empty asm keeps each read out of a branch that
+        // would otherwise be formed around the selects below.
+        const int32_t d0 = s - la0, d1 = s - la1;
+        float tv0 = tpre[min(max(d0, 0), a.len_pre - 1)];
+        float tv1 = tpre[min(max(d1, 0), a.len_pre - 1)];
+        asm volatile("" : "+v"(tv0), "+v"(tv1));
+        const float tp0 = (uint32_t)d0 < (uint32_t)a.len_pre ? tv0 : 0.0f;
+        const float tp1 = (uint32_t)d1 < (uint32_t)a.len_pre ? tv1 : 0.0f;
         // a weight whose synapse saw no recent arrival keeps its bits (the
         // trace is below half an ulp of it): no store
-        const float w0 = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la0), 0.0f, pre_f,
-                                 post_f, lr);
+        const float w0 = nc_stdp(s0.w, (uint32_t)s0.arr_inh & 1u, tp0, 0.0f, pre_f, post_f, lr);
         const bool ch0 = __float_as_uint(w0) != __float_as_uint(s0.w);
-        if (ch0) a.slots[j0].w = w0;
         bool ch1 = false;
+        float w1 = 0.0f;
         if (two) {
-          const float w1 = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, nc_lookup(tpre, a.len_pre, s - la1), 0.0f,
-                                   pre_f, post_f, lr);
+          w1 = nc_stdp(s1.w, (uint32_t)s1.arr_inh & 1u, tp1, 0.0f, pre_f, post_f, lr);
           ch1 = __float_as_uint(w1) != __float_as_uint(s1.w);
-          if (ch1) a.slots[j1].w = w1;
         }
+        if (ch0) a.slots[j0].w = w0;
+        if (ch1) a.slots[j1].w = w1;
         // counted per wave (a scalar), not per lane
         n_pw += (uint32_t)__builtin_popcountll(__ballot(ch0)) + (uint32_t)__builtin_popcountll(__ballot(ch1));
       }
@@ -667,7 +690,7 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   // after it); else, and for pending-list spikes, the record is read.  An
   // entry belongs to the lane that owns its slot: six ballots on the bits of
   // the target lane give each lane the set of entries that are its own.
-  const float tr_t = nc_lookup(tpost, a.len_post, s - lf);
+  const float tr_t = (!fire && tr_in) ? tr_tab : 0.0f;
   uint32_t arrw = 0;  // which of mask words 0-3 took arrivals
   // batch 0: the bucket's entries; batch b > 0: pending entries 64 (b - 1) ..
   const uint32_t n_batches = 1u + (pend_n + 63u) / 64u;
@@ -806,20 +829,16 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   }
 }
 
-// The trace tables in LDS when the launch shape asks for it (once per
-// workgroup), else through the caches.
+// The synapse-trace table of post-side STDP in LDS when the launch shape asks
+// for it (once per workgroup), else through the caches.  (The target-trace
+// table is read once per lane, with the prologue's loads, from global memory.)
 template <int WPB>
-__device__ __forceinline__ void nc_trace_tables(const nc_step_args& a, const float** tpre, const float** tpost) {
-  extern __shared__ float sh_tab[];  // trace tables: pre then post
-  *tpre = a.t_pre;
-  *tpost = a.t_post;
-  if (a.tabs_in_lds) {
-    for (int32_t i = threadIdx.x; i < a.len_pre; i += 64 * WPB) sh_tab[i] = a.t_pre[i];
-    for (int32_t i = threadIdx.x; i < a.len_post; i += 64 * WPB) sh_tab[a.len_pre + i] = a.t_post[i];
-    __syncthreads();
-    *tpre = sh_tab;
-    *tpost = sh_tab + a.len_pre;
-  }
+__device__ __forceinline__ const float* nc_trace_tables(const nc_step_args& a) {
+  extern __shared__ float sh_tab[];  // the pre-trace table
+  if (!a.tabs_in_lds) return a.t_pre;
+  for (int32_t i = threadIdx.x; i < a.len_pre; i += 64 * WPB) sh_tab[i] = a.t_pre[i];
+  __syncthreads();
+  return sh_tab;
 }
 
 // The neuron phase of one step for every active slice, one wave per slice,
@@ -828,9 +847,8 @@ template <int WPB>
 __global__ __launch_bounds__(64 * WPB) void k_neuron(nc_step_args a) {
   const uint32_t blk = nc_block_id(a.xcd_swizzle);
   const uint32_t wave = __builtin_amdgcn_readfirstlane(blk * WPB + (threadIdx.x >> 6));
-  const float *tpre, *tpost;
-  nc_trace_tables<WPB>(a, &tpre, &tpost);
-  nc_neuron_slice<WPB>(a, blk % a.ctr_rows, wave, (int32_t)*a.chunk_off + a.chunk_i, tpre, tpost);
+  const float* tpre = nc_trace_tables<WPB>(a);
+  nc_neuron_slice<WPB>(a, blk % a.ctr_rows, wave, (int32_t)*a.chunk_off + a.chunk_i, tpre);
 }
 
 // Synapse::fire (src/NeuCor.cpp:727-738) on every outgoing synapse of every
@@ -1016,10 +1034,9 @@ __global__ __launch_bounds__(64 * WPB) void k_step(nc_step_args a, uint32_t* bar
   const uint32_t nwaves = gridDim.x * WPB;
   const uint32_t w0 = __builtin_amdgcn_readfirstlane(blockIdx.x * WPB + (threadIdx.x >> 6));
   const int32_t ci = (int32_t)*a.chunk_off + a.chunk_i;
-  const float *tpre, *tpost;
-  nc_trace_tables<WPB>(a, &tpre, &tpost);
+  const float* tpre = nc_trace_tables<WPB>(a);
   // every wave of the grid runs the same number of slice passes
-  for (uint32_t b = 0; b < a.n_active_slices; b += nwaves) nc_neuron_slice<WPB>(a, blk, b + w0, ci, tpre, tpost);
+  for (uint32_t b = 0; b < a.n_active_slices; b += nwaves) nc_neuron_slice<WPB>(a, blk, b + w0, ci, tpre);
   nc_grid_barrier(bar, ((uint32_t)ci + 1u) * gridDim.x, bar + 32);
   nc_fanout_wave(a, blk, w0 % NC_FLISTS, w0 / NC_FLISTS, nwaves / NC_FLISTS, ci);
 }
