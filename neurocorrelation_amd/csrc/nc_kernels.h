@@ -451,6 +451,28 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   uint32_t* const mrow = a.amask + rec.mask_off + lane;
   // The bucket's entries are taken up after the fire.
   const uint32_t b_take = b_n < NC_BCAP ? b_n : NC_BCAP;
+  // The weight now of the bucket entry this lane holds, read at its slot
+  // record with the mask words instead of after the fire, when the target
+  // fired since the source did (as of the previous step) and no host write
+  // came after the fire (the entry's sign class then still holds; after a
+  // host write the record is read below, as for a target that fires now).
+  // Nothing changes that weight before the entry is taken up, except
+  // post-side STDP of a target that fires in this step, whose trace at s is
+  // 0: its pre-side STDP then does not use the weight.  The read is not
+  // under a branch: another lane reads its entry's own weight word, so that
+  // e_w is the weight for every entry of the bucket.
+  const uint32_t e_rel = be.x & 0xFFFFFFu;
+  const uint32_t e_tl = (e_rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
+  const int32_t e_sf = s - (int32_t)(((be.x >> 24) & 63u) + 1u);
+  const int32_t e_lf = __shfl(lf, (int)e_tl);
+  const bool e_stale = lane < b_take && e_lf > e_sf && e_sf >= nd.whost;
+  const float* const e_wp =
+      e_stale ? &a.slots[nc_slot_of(rec.slot_off, e_tl,
+                                    ((e_rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (e_rel & (NC_CHUNK - 1u)))]
+                     .w
+              : reinterpret_cast<const float*>(&a.bent[bi * NC_BCAP + lane].y);
+  const float e_w = *e_wp;
+  n_arec += e_stale ? 1u : 0u;
   // The lane's cache of (depolarisation, arrival) of its first NC_ACACHE
   // active slots in slot order (an arrival from its low byte: one of the
   // last R steps).
@@ -584,6 +606,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
 
   uint32_t n_post = 0;
   const unsigned long long fm = __ballot(fire);
+  // the slice's fan-out records (lane 0), written after post-side STDP
+  uint32_t f_at = 0u, f_rx = 0u, f_ry = 0u, f_tag = 0u, f_n = 0u, f_b = 0u;
   {
     // This step's lists get the fired slices' fan-out records; the other
     // parity's lists, which served the previous step's fan-out, are cleared
@@ -609,10 +633,15 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
         ry = orow.y;
         tag |= NC_FSINGLE;
       }
-      const uint32_t at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], n_frec);
-      for (uint32_t i = 0; i < n_frec; ++i)
-        a.fired[b * a.fcap + at + i] =
-            make_uint4(rx, ry, (uint32_t)s, tag | (i << 24) | ((n_frec - 1u) << 28));
+      // The places now, the records after post-side STDP: the atomic's
+      // return (many waves append to each list) then overlaps the row passes'
+      // reads instead of holding them back.
+      f_at = atomicAdd(&a.fired_cnt[(par * NC_FLISTS + b) * NC_FCNT_STRIDE], n_frec);
+      f_rx = rx;
+      f_ry = ry;
+      f_tag = tag | ((n_frec - 1u) << 28);
+      f_n = n_frec;
+      f_b = b;
       if (a.ctr) atomicAdd(&a.ctr[(size_t)blk * NC_NCTR + 11], (unsigned long long)n_frec);
     }
   }
@@ -677,6 +706,9 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   }
+  if (lane == 0)
+    for (uint32_t i = 0; i < f_n; ++i)
+      a.fired[f_b * a.fcap + f_at + i] = make_uint4(f_rx, f_ry, (uint32_t)s, f_tag | (i << 24));
 
   // Spikes arriving at s (Synapse::run, :718-726): the bucket's entries,
   // which carry the weight at the fire (hence the latched depolarisation),
@@ -711,18 +743,16 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
     const uint32_t tl = (rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
     const uint32_t kk = ((rel >> (6 + NC_CHUNK_LOG)) << NC_CHUNK_LOG) | (rel & (NC_CHUNK - 1u));
     // The weight now, the sign class and the latched depolarisation of each
-    // entry, found by the lane that holds the entry, all in one round: the
-    // entry's own while nothing can have changed the weight since the fire
-    // (the target has not fired since, and no host write came after it),
-    // else the slot record's (read only when the weight is used: pre-side
-    // STDP, or a pending-list spike).
+    // entry, found by the lane that holds the entry.  A bucket entry's weight
+    // was read with the mask words, unless a host write came after its fire:
+    // then, as for a pending-list spike, the slot record is read now, all in
+    // one round (the sign class too, which the write may have changed).  A
+    // target whose trace at s is 0 takes no pre-side STDP and does not use
+    // the weight.
     const int32_t s_fire = s - (int32_t)((info & 63u) + 1u);
-    const int32_t lf_t = __shfl(lf, (int)tl);
-    const float tr_tt = __int_as_float(__shfl(__float_as_int(tr_t), (int)tl));
-    const bool fresh = known && lf_t <= s_fire && s_fire >= nd.whost;
-    uint32_t w_now = wbits, kinh = kk | ((info >> 7) << 31);
+    uint32_t w_now = known ? __float_as_uint(e_w) : 0u, kinh = kk | ((info >> 7) << 31);
     uint32_t dkv = __float_as_uint(a.depol_k * __uint_as_float(wbits));  // Synapse::fire (:730-732), as fan-out made it
-    if (ev && !fresh && (tr_tt != 0.0f || !known)) {
+    if (ev && (!known || s_fire < nd.whost)) {
       const nc_slot r = a.slots[nc_slot_of(rec.slot_off, tl, kk)];
       ++n_arec;
       w_now = __float_as_uint(r.w);
