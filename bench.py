@@ -25,12 +25,21 @@ sys.path.insert(0, ROOT)
 METRIC = "synapse updates/s (% HBM roofline); served sims/s + p50/p99 vs plain"
 HBM_PEAK_GBS = 8000.0
 TIMING_EVERY = 10  # kernel events on every 10th step's launches at most; at least 5 timed launches per kernel
+
+
+def timing_every(steps):
+    """The spacing of the steps whose launches carry kernel events: every
+    TIMING_EVERY-th step, closer on short runs so that at least 5 launches
+    per kernel are timed."""
+    return min(TIMING_EVERY, max(1, steps // 5))
+
+
 # Launch shapes that the live tuner chose for a plain workload on MI355X, used
 # by runs too short to tune (under 1000 timed steps). plain-1m: 1 slice per
-# workgroup with the XCD-ordered remap, 1.437 ms per 16-step trial after the
-# warm-up against 1.488 ms for the engine's default (4 slices per workgroup, no
+# workgroup with the XCD-ordered remap, 1.367 ms per 16-step trial after the
+# warm-up against 1.421 ms for the engine's default (4 slices per workgroup, no
 # remap); 20 timed steps from step 6, three runs each: 73.1 against 75.6 us per
-# step.
+# step (an earlier build).
 TUNED_LAUNCH = {"plain-1m": {"waves_per_block": 1, "xcd_swizzle": 1}}
 
 
@@ -525,7 +534,7 @@ def main():
         print("[bench] tuned launch %s (%.2fs; default shape %.3f ms, chosen %.3f ms per trial)" %
               (launch, tuning["s"], float(ms[cands.index(dict(waves_per_block=4, xcd_swizzle=0, stage_tables=0))]),
                float(ms.min())), file=sys.stderr, flush=True)
-    every = min(TIMING_EVERY, max(1, args.steps // 5))
+    every = timing_every(args.steps)
     if not args.no_kernel_timing:
         # start/stop events carried by the neuron-step and fan-out dispatches
         # of every `every`-th step: each step that carries them takes about

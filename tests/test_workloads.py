@@ -86,3 +86,24 @@ def test_bench_measures_one_gpu_only():
                            capture_output=True, text=True, timeout=120)
         assert p.returncode == 2, p.stderr
         assert "one GPU" in p.stderr and p.stdout == ""
+
+
+def test_bench_tuned_shapes_and_event_spacing():
+    """The benchmark's table of tuned launch shapes names real plain
+    workloads and only options that Simulation.set_launch takes; kernel
+    events ride on at least 5 launches per kernel and on at most every 10th
+    step."""
+    import inspect
+    from neurocorrelation_amd.engine import Simulation
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    opts = set(inspect.signature(Simulation.set_launch).parameters) - {"self"}
+    for name, shape in bench.TUNED_LAUNCH.items():
+        assert name in W.WORKLOADS, name
+        assert set(shape) <= opts, (name, shape)
+        assert shape.get("waves_per_block", 4) in (1, 2, 4, 8)
+    for steps, want in ((20, 4), (5, 1), (3, 1), (49, 9), (50, 10), (200, 10), (10_000, 10)):
+        every = bench.timing_every(steps)
+        assert every == want, (steps, every)
+        assert steps < 5 or (steps + every - 1) // every >= 5
