@@ -458,8 +458,8 @@ __device__ __forceinline__ void nc_neuron_slice(const nc_step_args& a, uint32_t 
   // host write the record is read below, as for a target that fires now).
   // Nothing changes that weight before the entry is taken up, except
   // post-side STDP of a target that fires in this step, whose trace at s is
-  // 0: its pre-side STDP then does not use the weight.  The read is not
-  // under a branch: another lane reads its entry's own weight word, so that
+  // 0: its pre-side STDP then does not use the weight.  No branch: a lane
+  // with no such entry reads the weight word of its own entry place, so that
   // e_w is the weight for every entry of the bucket.
   const uint32_t e_rel = be.x & 0xFFFFFFu;
   const uint32_t e_tl = (e_rel & (64u * NC_CHUNK - 1u)) >> NC_CHUNK_LOG;
